@@ -1,0 +1,571 @@
+// cse_engine.hip -- host side of the engine: context, device buffers,
+// launches and the C ABI declared in include/cse.h.  Kernels live in
+// cse_kernels.hip (same translation unit).
+#include "cse_kernels.hip"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/cse.h"
+#include "cse_host.h"
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct cse_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    int cus = 0;
+    char name[128] = {0};
+    char err[512] = {0};
+    DevBuf s1, s2, out, recs, terms, zsum, kahan, hist, ips, queue, itemoff, inst, projs, thr, sexp, L,
+        idx, errflag, recoff;
+    bool thr_ready = false;
+    int blocks_per_cu = 8;  // persistent grid: blocks of 4 waves per CU
+};
+
+namespace {
+
+int fail(cse_ctx *c, int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(ctx, CSE_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                              \
+    } while (0)
+
+int ensure(cse_ctx *ctx, DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return CSE_OK;
+    if (b.p) HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 4;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, CSE_ERR_NOMEM, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+    }
+    b.cap = want;
+    return CSE_OK;
+}
+
+#define ENSURE(buf, bytes)                            \
+    do {                                              \
+        int rc_ = ensure(ctx, ctx->buf, (bytes));     \
+        if (rc_) return rc_;                          \
+    } while (0)
+
+template <typename T>
+T *dp(DevBuf &b) { return (T *)b.p; }
+
+int zero64(cse_ctx *ctx, void *p, long long words) {
+    if (words <= 0) return CSE_OK;
+    int blocks = (int)((words + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero64, dim3(blocks), dim3(256), 0, ctx->stream, (unsigned long long *)p, words);
+    HIPCHK(hipGetLastError());
+    return CSE_OK;
+}
+
+int ensure_thresholds(cse_ctx *ctx) {
+    if (ctx->thr_ready) return CSE_OK;
+    std::vector<uint64_t> thr(65 * 65, 0);
+    for (int n = 1; n <= 64; n++) cse_dimension_thresholds(n, &thr[65 * n]);
+    ENSURE(thr, thr.size() * 8);
+    HIPCHK(hipMemcpyAsync(ctx->thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->thr_ready = true;
+    return CSE_OK;
+}
+
+DevProjector to_dev(const cse_projector *P) {
+    DevProjector d;
+    memset(&d, 0, sizeof(d));
+    if (!P) return d;
+    d.nstabs = P->nstabs;
+    for (int g = 0; g < P->nstabs && g < 64; g++) {
+        d.phase[g] = P->phase[g] & 3;
+        d.xs[g] = P->xs[g];
+        d.zs[g] = P->zs[g];
+    }
+    return d;
+}
+
+int check_projector(cse_ctx *ctx, const cse_projector *P) {
+    if (!P) return fail(ctx, CSE_ERR_ARG, "projector is NULL");
+    if (P->nqubits < 1 || P->nqubits > CSE_NMAX)
+        return fail(ctx, CSE_ERR_ARG, "projector width t=%d outside [1, %d]", P->nqubits, CSE_NMAX);
+    if (P->nstabs < 1 || P->nstabs > CSE_NMAX)
+        return fail(ctx, CSE_ERR_ARG, "projector has %d generators (need 1..%d)", P->nstabs, CSE_NMAX);
+    uint64_t m = P->nqubits == 64 ? ~0ull : ((1ull << P->nqubits) - 1);
+    for (int g = 0; g < P->nstabs; g++)
+        if ((P->xs[g] & ~m) || (P->zs[g] & ~m))
+            return fail(ctx, CSE_ERR_ARG, "generator %d has bits beyond t=%d", g, P->nqubits);
+    return CSE_OK;
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return (double)ms;
+}
+
+int default_chunk(long long chi) {
+    long long c = 256;
+    if (chi < c) c = chi;
+    return (int)(c < 1 ? 1 : c);
+}
+
+// Persistent grid for k_sweep
+int sweep_grid(cse_ctx *ctx, long long nitems) {
+    long long g = (long long)ctx->cus * ctx->blocks_per_cu;
+    long long need = (nitems + 3) / 4;
+    if (need < g) g = need;
+    return (int)(g < 1 ? 1 : g);
+}
+
+// Records for instances [0, ninst): mode 0 random states (sample indices
+// first + i), mode 1 projected decomposition states.  Then the sweep.
+struct Plan {
+    std::vector<InstDesc> inst;
+    std::vector<long long> rec_off;   // ninst + 1
+    std::vector<long long> item_off;  // ninst + 1
+    std::vector<DevProjector> projs;
+    long long nrec = 0, nitems = 0, nterms = 0;
+};
+
+int upload_plan(cse_ctx *ctx, const Plan &pl) {
+    const int ninst = (int)pl.inst.size();
+    ENSURE(inst, sizeof(InstDesc) * ninst);
+    ENSURE(itemoff, sizeof(long long) * (ninst + 1));
+    ENSURE(recoff, sizeof(long long) * (ninst + 1));
+    ENSURE(projs, sizeof(DevProjector) * ninst);
+    HIPCHK(hipMemcpyAsync(ctx->inst.p, pl.inst.data(), sizeof(InstDesc) * ninst, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->itemoff.p, pl.item_off.data(), sizeof(long long) * (ninst + 1),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->recoff.p, pl.rec_off.data(), sizeof(long long) * (ninst + 1),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->projs.p, pl.projs.data(), sizeof(DevProjector) * ninst, hipMemcpyHostToDevice,
+                          ctx->stream));
+    return CSE_OK;
+}
+
+int run_records(cse_ctx *ctx, const Plan &pl, int mode, uint64_t seed, long long first) {
+    const int ninst = (int)pl.inst.size();
+    ENSURE(recs, (size_t)REC_WORDS * 8 * (size_t)pl.nrec);
+    ENSURE(sexp, 4 * (size_t)pl.nrec);
+    ENSURE(errflag, 8);
+    int rc0 = zero64(ctx, ctx->errflag.p, 1);
+    if (rc0) return rc0;
+    const long long waves = pl.nrec;
+    const long long blocks = (waves + 3) / 4;
+    if (blocks > 0) {
+        hipLaunchKernelGGL(k_records, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, mode, ninst,
+                           dp<long long>(ctx->recoff), dp<InstDesc>(ctx->inst), dp<DevProjector>(ctx->projs),
+                           dp<uint64_t>(ctx->thr), seed, first, pl.nrec, dp<uint64_t>(ctx->recs),
+                           dp<int32_t>(ctx->sexp), dp<int>(ctx->errflag));
+        HIPCHK(hipGetLastError());
+    }
+    return CSE_OK;
+}
+
+int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
+    const int ninst = (int)pl.inst.size();
+    ENSURE(queue, 8);
+    ENSURE(ips, 8 * (size_t)ninst);
+    ENSURE(hist, 8 * (size_t)HIST_STRIDE * ninst);
+    if (mode == 0) {
+        ENSURE(zsum, 32 * (size_t)pl.nrec);
+        ENSURE(kahan, 16 * (size_t)pl.nrec);
+        int rc = zero64(ctx, ctx->zsum.p, 4 * pl.nrec);
+        if (!rc) rc = zero64(ctx, ctx->kahan.p, 2 * pl.nrec);
+        if (rc) return rc;
+    } else {
+        ENSURE(kahan, 16 * (size_t)ninst);
+        int rc = zero64(ctx, ctx->kahan.p, 2 * (long long)ninst);
+        if (rc) return rc;
+    }
+    int rc = zero64(ctx, ctx->queue.p, 1);
+    if (!rc) rc = zero64(ctx, ctx->ips.p, ninst);
+    if (!rc) rc = zero64(ctx, ctx->hist.p, (long long)HIST_STRIDE * ninst);
+    if (rc) return rc;
+    if (want_terms) ENSURE(terms, 4 * (size_t)(pl.nterms > 0 ? pl.nterms : 1));
+    SweepArgs A;
+    A.mode = mode;
+    A.ninst = ninst;
+    A.nitems = pl.nitems;
+    A.item_off = dp<long long>(ctx->itemoff);
+    A.inst = dp<InstDesc>(ctx->inst);
+    A.recs = dp<uint64_t>(ctx->recs);
+    A.queue = dp<unsigned long long>(ctx->queue);
+    A.terms = want_terms ? dp<int32_t>(ctx->terms) : nullptr;
+    A.zsum = mode == 0 ? dp<long long>(ctx->zsum) : nullptr;
+    A.kahan = dp<double>(ctx->kahan);
+    A.hist = dp<unsigned long long>(ctx->hist);
+    A.hist_stride = HIST_STRIDE;
+    A.ips = dp<unsigned long long>(ctx->ips);
+    A.err = dp<int>(ctx->errflag);
+    const int grid = sweep_grid(ctx, pl.nitems);
+    hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(256), 0, ctx->stream, A);
+    HIPCHK(hipGetLastError());
+    return CSE_OK;
+}
+
+InstDesc make_inst(int t, int k, int exact, const uint64_t *L, int chunk) {
+    InstDesc d;
+    memset(&d, 0, sizeof(d));
+    d.n = t;
+    d.k = exact ? 0 : k;
+    d.exact = exact ? 1 : 0;
+    d.chi = exact ? (1ll << ((t + 1) / 2)) : (1ll << k);
+    d.chunk = chunk > 0 ? chunk : default_chunk(d.chi);
+    if (d.chunk > d.chi) d.chunk = (int)d.chi;
+    d.nchunks = (d.chi + d.chunk - 1) / d.chunk;
+    if (!exact)
+        for (int r = 0; r < k; r++) d.L[r] = L[r];
+    return d;
+}
+
+int check_decomp(cse_ctx *ctx, int t, int k, int exact, const uint64_t *L) {
+    if (exact) {
+        if ((t + 1) / 2 > 40) return fail(ctx, CSE_ERR_ARG, "exact decomposition 2^%d too large", (t + 1) / 2);
+        return CSE_OK;
+    }
+    if (k < 0 || k > 40) return fail(ctx, CSE_ERR_ARG, "k=%d outside [0, 40]", k);
+    if (k > 0 && !L) return fail(ctx, CSE_ERR_ARG, "L is NULL");
+    uint64_t m = t == 64 ? ~0ull : ((1ull << t) - 1);
+    for (int r = 0; r < k; r++)
+        if (L[r] & ~m) return fail(ctx, CSE_ERR_ARG, "row %d of L has bits beyond t=%d", r, t);
+    return CSE_OK;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+int cse_create(int device, cse_ctx **out) {
+    if (!out) return CSE_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return CSE_ERR_HIP;
+    }
+    if (device < 0 || device >= count) return CSE_ERR_ARG;
+    cse_ctx *ctx = new cse_ctx();
+    ctx->device = device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ctx->e0));
+    HIPCHK(hipEventCreate(&ctx->e1));
+    HIPCHK(hipEventCreate(&ctx->e2));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    ctx->cus = prop.multiProcessorCount;
+    snprintf(ctx->name, sizeof(ctx->name), "%s (%s)", prop.name, prop.gcnArchName);
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        int rc = fail(ctx, CSE_ERR_HIP, "device %d is %s; this engine is built for gfx950", device, prop.gcnArchName);
+        cse_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return CSE_OK;
+}
+
+void cse_destroy(cse_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->s1, &ctx->s2, &ctx->out, &ctx->recs, &ctx->terms, &ctx->zsum, &ctx->kahan,
+                      &ctx->hist, &ctx->ips, &ctx->queue, &ctx->itemoff, &ctx->inst, &ctx->projs, &ctx->thr,
+                      &ctx->sexp, &ctx->L, &ctx->idx, &ctx->errflag, &ctx->recoff};
+    for (DevBuf *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    if (ctx->e0) (void)hipEventDestroy(ctx->e0);
+    if (ctx->e1) (void)hipEventDestroy(ctx->e1);
+    if (ctx->e2) (void)hipEventDestroy(ctx->e2);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *cse_error(const cse_ctx *ctx) { return ctx ? ctx->err : "no context"; }
+
+int cse_device_info(cse_ctx *ctx, int *cus, char *name, int name_len) {
+    if (!ctx) return CSE_ERR_ARG;
+    if (cus) *cus = ctx->cus;
+    if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", ctx->name);
+    return CSE_OK;
+}
+
+int cse_inner_products(cse_ctx *ctx, int n, size_t npairs, const uint64_t *s1, const uint64_t *s2,
+                       int32_t *out, int one_launch_per_pair, double *ms) {
+    if (!ctx) return CSE_ERR_ARG;
+    if (n < 1 || n > CSE_NMAX) return fail(ctx, CSE_ERR_ARG, "n=%d outside [1, %d]", n, CSE_NMAX);
+    if (npairs == 0) {
+        if (ms) *ms = 0;
+        return CSE_OK;
+    }
+    if (!s1 || !s2 || !out) return fail(ctx, CSE_ERR_ARG, "NULL buffer");
+    const size_t words = (size_t)CSE_STATE_WORDS(n);
+    for (size_t i = 0; i < npairs; i++) {
+        const uint64_t *a = s1 + words * i, *b = s2 + words * i;
+        if ((int)(a[0] & 0xff) != n || (int)(b[0] & 0xff) != n)
+            return fail(ctx, CSE_ERR_ARG, "pair %zu: header n does not match n=%d", i, n);
+        if ((int)((a[0] >> 8) & 0xff) > n || (int)((b[0] >> 8) & 0xff) > n)
+            return fail(ctx, CSE_ERR_ARG, "pair %zu: k > n", i);
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    ENSURE(s1, words * 8 * npairs);
+    ENSURE(s2, words * 8 * npairs);
+    ENSURE(out, 12 * npairs);
+    HIPCHK(hipMemcpyAsync(ctx->s1.p, s1, words * 8 * npairs, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->s2.p, s2, words * 8 * npairs, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
+    if (one_launch_per_pair) {
+        for (size_t i = 0; i < npairs; i++) {
+            hipLaunchKernelGGL(k_ip_pairs, dim3(1), dim3(64), 0, ctx->stream, dp<uint64_t>(ctx->s1) + words * i,
+                               dp<uint64_t>(ctx->s2) + words * i, n, 1, dp<int32_t>(ctx->out) + 3 * i);
+        }
+    } else {
+        const size_t blocks = (npairs + 3) / 4;
+        hipLaunchKernelGGL(k_ip_pairs, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, dp<uint64_t>(ctx->s1),
+                           dp<uint64_t>(ctx->s2), n, (int)npairs, dp<int32_t>(ctx->out));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->e1, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, ctx->out.p, 12 * npairs, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ms) *ms = elapsed(ctx->e0, ctx->e1);
+    return CSE_OK;
+}
+
+int cse_prep_states(cse_ctx *ctx, int exact, int t, const uint64_t *L, int k, const uint64_t *idx, size_t count,
+                    uint64_t *out) {
+    if (!ctx) return CSE_ERR_ARG;
+    if (t < 1 || t > CSE_NMAX) return fail(ctx, CSE_ERR_ARG, "t=%d outside [1, %d]", t, CSE_NMAX);
+    if (count == 0) return CSE_OK;
+    int rc = check_decomp(ctx, t, k, exact, L);
+    if (rc) return rc;
+    if (!idx || !out) return fail(ctx, CSE_ERR_ARG, "NULL buffer");
+    const uint64_t chi = exact ? (1ull << ((t + 1) / 2)) : (1ull << k);
+    for (size_t c = 0; c < count; c++)
+        if (idx[c] >= chi) return fail(ctx, CSE_ERR_ARG, "index %llu >= chi", (unsigned long long)idx[c]);
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t words = (size_t)CSE_STATE_WORDS(t);
+    ENSURE(L, 8 * 64);
+    ENSURE(idx, 8 * count);
+    ENSURE(s1, 8 * words * count);
+    if (!exact && k > 0) HIPCHK(hipMemcpyAsync(ctx->L.p, L, 8 * (size_t)k, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->idx.p, idx, 8 * count, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_prep_states, dim3((unsigned)((count + 3) / 4)), dim3(256), 0, ctx->stream, exact, t,
+                       dp<uint64_t>(ctx->L), k, dp<uint64_t>(ctx->idx), (int)count, dp<uint64_t>(ctx->s1));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, ctx->s1.p, 8 * words * count, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CSE_OK;
+}
+
+int cse_random_states(cse_ctx *ctx, int n, uint64_t seed, int64_t first, size_t count, const cse_projector *P,
+                      uint64_t *out, int32_t *sexp) {
+    if (!ctx) return CSE_ERR_ARG;
+    if (n < 1 || n > CSE_NMAX) return fail(ctx, CSE_ERR_ARG, "n=%d outside [1, %d]", n, CSE_NMAX);
+    if (count == 0) return CSE_OK;
+    if (P) {
+        int rc = check_projector(ctx, P);
+        if (rc) return rc;
+        if (P->nqubits != n) return fail(ctx, CSE_ERR_ARG, "projector width %d != n %d", P->nqubits, n);
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_thresholds(ctx);
+    if (rc) return rc;
+    const size_t words = (size_t)CSE_STATE_WORDS(n);
+    ENSURE(s1, 8 * words * count);
+    ENSURE(sexp, 4 * count);
+    ENSURE(errflag, 8);
+    rc = zero64(ctx, ctx->errflag.p, 1);
+    if (rc) return rc;
+    DevProjector dpj = to_dev(P);
+    hipLaunchKernelGGL(k_random_states, dim3((unsigned)((count + 3) / 4)), dim3(256), 0, ctx->stream, n,
+                       dp<uint64_t>(ctx->thr) + 65 * n, seed, (long long)first, (int)count, dpj, P ? 1 : 0,
+                       out ? dp<uint64_t>(ctx->s1) : nullptr, dp<int32_t>(ctx->sexp), dp<int>(ctx->errflag));
+    HIPCHK(hipGetLastError());
+    int errv = 0;
+    HIPCHK(hipMemcpyAsync(&errv, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (out) HIPCHK(hipMemcpyAsync(out, ctx->s1.p, 8 * words * count, hipMemcpyDeviceToHost, ctx->stream));
+    if (sexp) HIPCHK(hipMemcpyAsync(sexp, ctx->sexp.p, 4 * count, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (errv) return fail(ctx, CSE_ERR_RNG, "random state draw exceeded its shrink cap");
+    return CSE_OK;
+}
+
+int cse_sampled_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k, int exact,
+                          uint64_t seed, int64_t first, int64_t count, int chunk, cse_sampled_result *res) {
+    if (!ctx || !res) return CSE_ERR_ARG;
+    int rc = check_projector(ctx, P);
+    if (rc) return rc;
+    const int t = P->nqubits;
+    rc = check_decomp(ctx, t, k, exact, L);
+    if (rc) return rc;
+    if (count < 1) return fail(ctx, CSE_ERR_ARG, "count=%lld", (long long)count);
+    if (count > (1ll << 24)) return fail(ctx, CSE_ERR_ARG, "count=%lld above 2^24 per call", (long long)count);
+    HIPCHK(hipSetDevice(ctx->device));
+    rc = ensure_thresholds(ctx);
+    if (rc) return rc;
+
+    Plan pl;
+    pl.inst.push_back(make_inst(t, k, exact, L, chunk));
+    pl.projs.push_back(to_dev(P));
+    pl.rec_off = {0, count};
+    pl.nrec = count;
+    pl.nitems = count * pl.inst[0].nchunks;
+    pl.item_off = {0, pl.nitems};
+    pl.nterms = count * pl.inst[0].chi;
+    if (res->terms && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
+    rc = upload_plan(ctx, pl);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
+    rc = run_records(ctx, pl, 0, seed, first);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->e1, ctx->stream));
+    rc = run_sweep(ctx, pl, 0, res->terms != nullptr);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
+
+    std::vector<long long> z(4 * (size_t)count);
+    std::vector<double> kz(2 * (size_t)count);
+    std::vector<int32_t> se((size_t)count);
+    unsigned long long nips = 0;
+    int errv = 0;
+    HIPCHK(hipMemcpyAsync(z.data(), ctx->zsum.p, 32 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(kz.data(), ctx->kahan.p, 16 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(se.data(), ctx->sexp.p, 4 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&nips, ctx->ips.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&errv, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (res->terms)
+        HIPCHK(hipMemcpyAsync(res->terms, ctx->terms.p, 4 * (size_t)pl.nterms, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (errv & 1) return fail(ctx, CSE_ERR_RNG, "random state draw exceeded its shrink cap");
+    if (errv & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
+
+    res->prep_ms = elapsed(ctx->e0, ctx->e1);
+    res->kernel_ms = elapsed(ctx->e1, ctx->e2);
+    res->ips = (int64_t)nips;
+    res->vanished = 0;
+    for (int64_t s = 0; s < count; s++)
+        if (se[s] < 0) res->vanished++;
+    cse_sampled_estimators(t, count, z.data(), se.data(), kz.data(), &res->mean, &res->mean_kahan);
+    if (res->zsum) memcpy(res->zsum, z.data(), 32 * (size_t)count);
+    if (res->sexp) memcpy(res->sexp, se.data(), 4 * (size_t)count);
+    if (res->kahan) memcpy(res->kahan, kz.data(), 16 * (size_t)count);
+    return CSE_OK;
+}
+
+static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chunk, cse_exact_result *res,
+                       int32_t *terms_out, double *kernel_ms) {
+    Plan pl;
+    pl.rec_off.push_back(0);
+    pl.item_off.push_back(0);
+    for (int i = 0; i < ninst; i++) {
+        InstDesc d = make_inst(in[i].P.nqubits, in[i].k, in[i].exact, in[i].L, chunk);
+        d.rec_off = pl.nrec;
+        d.term_off = pl.nterms;
+        pl.nrec += d.chi;
+        pl.nitems += d.chi * d.nchunks;
+        pl.nterms += d.chi * (d.chi + 1) / 2;
+        pl.rec_off.push_back(pl.nrec);
+        pl.item_off.push_back(pl.nitems);
+        pl.inst.push_back(d);
+        pl.projs.push_back(to_dev(&in[i].P));
+    }
+    if (terms_out && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
+    int rc = ensure_thresholds(ctx);
+    if (!rc) rc = upload_plan(ctx, pl);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
+    rc = run_records(ctx, pl, 1, 0, 0);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->e1, ctx->stream));
+    rc = run_sweep(ctx, pl, 1, terms_out != nullptr);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
+    std::vector<unsigned long long> hist((size_t)HIST_STRIDE * ninst);
+    std::vector<double> kz(2 * (size_t)ninst);
+    std::vector<unsigned long long> nips((size_t)ninst);
+    int errv = 0;
+    HIPCHK(hipMemcpyAsync(&errv, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hist.data(), ctx->hist.p, 8 * hist.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(kz.data(), ctx->kahan.p, 16 * (size_t)ninst, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(nips.data(), ctx->ips.p, 8 * (size_t)ninst, hipMemcpyDeviceToHost, ctx->stream));
+    if (terms_out)
+        HIPCHK(hipMemcpyAsync(terms_out, ctx->terms.p, 4 * (size_t)pl.nterms, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (errv & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
+    const double kms = elapsed(ctx->e1, ctx->e2), tms = elapsed(ctx->e0, ctx->e2);
+    if (kernel_ms) *kernel_ms = kms;
+    for (int i = 0; i < ninst; i++) {
+        cse_exact_result *r = &res[i];
+        const int64_t *h = (const int64_t *)&hist[(size_t)HIST_STRIDE * i];
+        r->pmin = -128;
+        if (r->hist) memcpy(r->hist, h, 8 * (size_t)HIST_STRIDE);
+        cse_hist_value(h, -128, &r->re, &r->im);
+        r->value = hypot(r->re, r->im);
+        r->kahan_re = kz[2 * i];
+        r->kahan_im = kz[2 * i + 1];
+        r->terms = pl.inst[i].chi * (pl.inst[i].chi + 1) / 2;
+        r->ips = (int64_t)nips[i];
+        r->kernel_ms = kms;
+        r->total_ms = tms;
+    }
+    return CSE_OK;
+}
+
+int cse_exact_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k, int exact,
+                        cse_exact_result *res) {
+    if (!ctx || !res) return CSE_ERR_ARG;
+    int rc = check_projector(ctx, P);
+    if (rc) return rc;
+    rc = check_decomp(ctx, P->nqubits, k, exact, L);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    cse_instance in;
+    memset(&in, 0, sizeof(in));
+    in.P = *P;
+    in.exact = exact;
+    in.k = exact ? 0 : k;
+    if (!exact)
+        for (int r = 0; r < k; r++) in.L[r] = L[r];
+    return exact_batch(ctx, &in, 1, 0, res, nullptr, nullptr);
+}
+
+int cse_batch_exact(cse_ctx *ctx, const cse_instance *inst, int ninst, int chunk, cse_exact_result *res,
+                    int32_t *terms_out, double *kernel_ms) {
+    if (!ctx || !inst || !res || ninst < 1) return CSE_ERR_ARG;
+    for (int i = 0; i < ninst; i++) {
+        int rc = check_projector(ctx, &inst[i].P);
+        if (!rc) rc = check_decomp(ctx, inst[i].P.nqubits, inst[i].k, inst[i].exact, inst[i].L);
+        if (rc) return fail(ctx, rc, "instance %d: %s", i, ctx->err);
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    return exact_batch(ctx, inst, ninst, chunk, res, terms_out, kernel_ms);
+}
+
+}  // extern "C"

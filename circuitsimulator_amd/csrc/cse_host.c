@@ -1,0 +1,296 @@
+/*
+ * cse_host.c -- host-side helpers of the engine (plain C): counter-based RNG,
+ * the dimension law of random stabilizer states, exact-to-FP64 conversion of
+ * the exact accumulators, and decompose (libcirc/probability.c:307-415).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/cse.h"
+#include "cse_host.h"
+
+/* ------------------------------------------------------------- Philox */
+void cse_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = (uint32_t)p1;
+        c2 = n2;
+        c3 = (uint32_t)p0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static uint32_t rand32(uint64_t key, uint64_t id, uint32_t domain, uint64_t w)
+{
+    uint32_t ctr[4] = {(uint32_t)(w >> 2), (uint32_t)id, (uint32_t)(id >> 32), domain};
+    uint32_t k[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    uint32_t out[4];
+    cse_philox4x32_10(ctr, k, out);
+    return out[w & 3];
+}
+
+uint64_t cse_rand64(uint64_t key, uint64_t id, uint32_t domain, uint64_t w)
+{
+    return (uint64_t)rand32(key, id, domain, w) | ((uint64_t)rand32(key, id, domain, w + 1) << 32);
+}
+
+/* ------------------------------------------------ dimension law, eq. 79 */
+static double logeta(int d, int n)
+{
+    if (d == 0) return 0.0;
+    double prod = 0;
+    for (int a = 1; a <= d; a++) {
+        prod += log2(1 - pow(2, d - n - a));
+        prod -= log2(1 - pow(2, -a));
+    }
+    return (-d * (d + 1) / 2) + prod;
+}
+
+void cse_dimension_thresholds(int n, uint64_t *thr)
+{
+    double dist[CSE_NMAX + 1], cum[CSE_NMAX + 1];
+    double sum = 0;
+    for (int d = 0; d <= n; d++) {
+        dist[d] = pow(2, logeta(d, n));
+        sum += dist[d];
+    }
+    for (int d = 0; d <= n; d++) dist[d] /= sum;
+    for (int i = 0; i <= n; i++) {
+        cum[i] = 0;
+        for (int d = 0; d <= i; d++) cum[i] += dist[d];
+    }
+    for (int i = 0; i <= n; i++) cum[i] /= cum[n];
+    for (int i = 0; i <= n; i++) thr[i] = (uint64_t)floor(cum[i] * 9007199254740992.0);
+}
+
+/* ------------------------------------------------------- double-double */
+static cse_dd two_sum(double a, double b)
+{
+    double s = a + b;
+    double bb = s - a;
+    double e = (a - (s - bb)) + (b - bb);
+    cse_dd r = {s, e};
+    return r;
+}
+
+static cse_dd quick_two_sum(double a, double b)
+{
+    double s = a + b;
+    cse_dd r = {s, b - (s - a)};
+    return r;
+}
+
+cse_dd cse_dd_add(cse_dd a, cse_dd b)
+{
+    cse_dd s = two_sum(a.hi, b.hi);
+    cse_dd t = two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = quick_two_sum(s.hi, s.lo);
+    s.lo += t.lo;
+    return quick_two_sum(s.hi, s.lo);
+}
+
+cse_dd cse_dd_mul(cse_dd a, cse_dd b)
+{
+    double p = a.hi * b.hi;
+    double e = fma(a.hi, b.hi, -p);
+    e += a.hi * b.lo + a.lo * b.hi;
+    return quick_two_sum(p, e);
+}
+
+cse_dd cse_dd_from_i128(__int128 v)
+{
+    double hi = (double)v;
+    __int128 rest = v - (__int128)hi;
+    cse_dd r = {hi, (double)rest};
+    return quick_two_sum(r.hi, r.lo);
+}
+
+static cse_dd dd_scale(cse_dd a, int e)
+{
+    cse_dd r = {ldexp(a.hi, e), ldexp(a.lo, e)};
+    return r;
+}
+
+/* sqrt(2) to double-double precision */
+static const cse_dd SQRT2 = {1.4142135623730951, -9.667293313452913e-17};
+
+/* ------------------------------------------------ exact -> FP64 */
+void cse_hist_value(const int64_t *hist, int pmin, double *re, double *im)
+{
+    /* bin (P, M), P = -(b/8): count * 2^{P/2} w^M = count * 2^e * (1 or sqrt2) * (+-1 or +-i);
+     * w^M for odd M carries 1/sqrt2. */
+    cse_dd xr = {0, 0}, yr = {0, 0}, xi = {0, 0}, yi = {0, 0};
+    const int nb = (1 - pmin) * 8;
+    for (int b = 0; b < nb; b++) {
+        int64_t c = hist[b];
+        if (!c) continue;
+        const int P = -(b / 8), M = b % 8;
+        /* 2^{P/2} = 2^{floor(P/2)} sqrt2^{P&1}; odd M: (+-1 +- i)/sqrt2 */
+        const int podd = P & 1;
+        int e = P >> 1;
+        int rs, is;  /* signs of the real / imaginary unit parts */
+        static const int cre[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+        static const int cim[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+        rs = cre[M];
+        is = cim[M];
+        /* factor = sqrt2^{podd} * (M odd ? 1/sqrt2 : 1) = sqrt2^{podd - modd} */
+        int s2 = podd - (M & 1); /* -1, 0 or 1 */
+        int useq = 0;
+        if (s2 == -1) { e -= 1; useq = 1; } /* 1/sqrt2 = sqrt2 / 2 */
+        else if (s2 == 1) useq = 1;
+        cse_dd v = cse_dd_from_i128((__int128)c);
+        v = dd_scale(v, e);
+        cse_dd vr = {rs * v.hi, rs * v.lo}, vi = {is * v.hi, is * v.lo};
+        if (useq) {
+            yr = cse_dd_add(yr, vr);
+            yi = cse_dd_add(yi, vi);
+        } else {
+            xr = cse_dd_add(xr, vr);
+            xi = cse_dd_add(xi, vi);
+        }
+    }
+    cse_dd r = cse_dd_add(xr, cse_dd_mul(yr, SQRT2));
+    cse_dd i = cse_dd_add(xi, cse_dd_mul(yi, SQRT2));
+    *re = r.hi + r.lo;
+    *im = i.hi + i.lo;
+}
+
+void cse_sampled_estimators(int t, int64_t count, const long long *z, const int32_t *sexp, const double *kz,
+                            double *mean, double *mean_kahan)
+{
+    cse_dd X = {0, 0}, Y = {0, 0};
+    double kacc = 0, kcomp = 0;
+    for (int64_t s = 0; s < count; s++) {
+        if (sexp[s] < 0) continue;
+        __int128 c0 = z[4 * s], c1 = z[4 * s + 1], c2 = z[4 * s + 2], c3 = z[4 * s + 3];
+        __int128 A = c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3;
+        __int128 B = c0 * c1 - c0 * c3 + c1 * c2 + c2 * c3;
+        const int e = t - sexp[s] - 64;
+        X = cse_dd_add(X, dd_scale(cse_dd_from_i128(A), e));
+        Y = cse_dd_add(Y, dd_scale(cse_dd_from_i128(B), e));
+        /* FP32-Kahan path: |S|^2 of the FP32 sums, summed compensated in FP64 */
+        double kr = kz[2 * s], ki = kz[2 * s + 1];
+        double v = ldexp(kr * kr + ki * ki, t - sexp[s]);
+        double yk = v - kcomp;
+        double tk = kacc + yk;
+        kcomp = (tk - kacc) - yk;
+        kacc = tk;
+    }
+    cse_dd tot = cse_dd_add(X, cse_dd_mul(Y, SQRT2));
+    cse_dd inv = {1.0 / (double)count, 0};
+    /* division by count: count is an integer < 2^53, refine once */
+    double q = (tot.hi + tot.lo) / (double)count;
+    cse_dd prod = cse_dd_mul((cse_dd){q, 0}, (cse_dd){(double)count, 0});
+    cse_dd rem = cse_dd_add(tot, (cse_dd){-prod.hi, -prod.lo});
+    q += (rem.hi + rem.lo) / (double)count;
+    (void)inv;
+    *mean = q;
+    *mean_kahan = kacc / (double)count;
+}
+
+/* ------------------------------------------------------------- rank */
+int cse_gf2_rank(const uint64_t *rows, int k)
+{
+    uint64_t r[CSE_NMAX];
+    int n = 0;
+    for (int i = 0; i < k && i < CSE_NMAX; i++) r[n++] = rows[i];
+    int rank = 0;
+    for (int col = 0; col < 64 && rank < n; col++) {
+        int piv = -1;
+        for (int i = rank; i < n; i++)
+            if ((r[i] >> col) & 1ull) { piv = i; break; }
+        if (piv < 0) continue;
+        uint64_t tmp = r[piv]; r[piv] = r[rank]; r[rank] = tmp;
+        for (int i = 0; i < n; i++)
+            if (i != rank && ((r[i] >> col) & 1ull)) r[i] ^= r[rank];
+        rank++;
+    }
+    return rank;
+}
+
+/* ----------------------------------------------------------- decompose */
+#define CSE_DOM_L 2u
+
+int cse_decompose(int t, uint64_t seed, uint64_t *L, double *norm, int *exact, int *k, double fidbound,
+                  int rank, int fidelity, int forceL, int verbose, int quiet)
+{
+    if (!L || !norm || !exact || !k) return CSE_ERR_ARG;
+    if (t < 0 || t > CSE_NMAX) return CSE_ERR_ARG;
+    if (t == 0) {
+        *exact = 0;
+        *norm = 1;
+        return CSE_OK;
+    }
+    const double v = cos(M_PI / 8);
+    *norm = pow(2, floor((float)t / 2) / 2);
+    if (t % 2) *norm *= 2 * v;
+    if (*exact) return CSE_OK;
+
+    int forceK = 1;
+    if (*k <= 0) {
+        forceK = 0;
+        *k = (int)ceil(1 - 2 * t * log2(v) - log2(fidbound));
+        if (verbose) printf("Autopicking k = %d to achieve delta = %f.\n", *k, fidbound);
+    }
+    if (*k > t / 2 && !forceK && !forceL) {
+        if (verbose) printf("k > t/2. Reverting to exact decomposition.\n");
+        *exact = 1;
+        return CSE_OK;
+    }
+    if (*k > t) {
+        if (forceK && !quiet) printf("Can't have k > t. Setting k to %d.\n", t);
+        *k = t;
+    }
+    if (*k > 40) return CSE_ERR_ARG;
+    const uint64_t tm = t == 64 ? ~0ull : ((1ull << t) - 1);
+    double innerProduct = 0, Z_L = 0;
+    uint64_t draw = 0;
+    uint64_t w = 0;
+    while (innerProduct < 1 - fidbound || forceK) {
+        for (int r = 0; r < *k; r++) {
+            L[r] = cse_rand64(seed, draw, CSE_DOM_L, w) & tm;
+            w += 2;
+        }
+        draw++;
+        if (rank) {
+            if (cse_gf2_rank(L, *k) < *k) {
+                if (!quiet) printf("L has insufficient rank. Sampling again...\n");
+                continue;
+            }
+        }
+        if (fidelity) {
+            Z_L = 0;
+            for (uint64_t i = 0; i < (1ull << *k); i++) {
+                uint64_t x = 0;
+                for (int j = 0; j < *k; j++)
+                    if ((i >> (*k - 1 - j)) & 1ull) x ^= L[j];
+                int hamming = __builtin_popcountll(x);
+                Z_L += pow(2, -hamming / 2); /* integer division, probability.c:396 */
+            }
+            innerProduct = pow(2, *k) * pow(v, 2 * t) / Z_L;
+            if (forceK) {
+                printf("delta = 1 - <H^t|L>: %lf\n", 1 - innerProduct);
+                break;
+            } else if (innerProduct < 1 - fidbound) {
+                if (!quiet) printf("delta = 1 - <H^t|L>: %lf - Not good enough!\n", 1 - innerProduct);
+            } else {
+                if (!quiet) printf("delta = 1 - <H^t|L>: %lf\n", 1 - innerProduct);
+            }
+        } else {
+            break;
+        }
+    }
+    if (fidelity) *norm = sqrt(pow(2, *k) * Z_L);
+    return CSE_OK;
+}

@@ -1,0 +1,114 @@
+// cse_internal.h -- layouts shared by the kernels and the host side of the
+// engine (not part of the public C ABI).
+#pragma once
+#include <stdint.h>
+
+#ifndef DEV
+#define DEV __device__ __forceinline__
+#endif
+
+// Projector as kernel argument: generator g is i^phase[g] Z(zs[g]) X(xs[g]).
+struct DevProjector {
+    int nstabs;
+    int phase[64];
+    uint64_t xs[64];
+    uint64_t zs[64];
+};
+
+// Elimination-ready record of a projected state theta' (reduced row echelon
+// basis of its active rows), REC_WORDS uint64 words:
+//   w[0] = k | n << 8 | Q << 16 | sexp << 24 | vanished << 40
+//   w[1] = h, w[2] = active-row mask, w[3] = mask of pivot columns
+//   w[4 .. 68)     G rows
+//   w[68 .. 132)   J rows
+//   w[132 .. 164)  64 x uint32: D_x | pivot column << 8
+#define REC_WORDS 164
+
+// Histogram bins per instance: P in [-(64 + 64), 0], 8 phases.
+#define HIST_STRIDE (129 * 8)
+// Per-wave LDS histogram: p in [-64, 0], 8 phases.
+#define LH_BINS (65 * 8)
+
+struct InstDesc {
+    int n;              // state dimension t
+    int k;              // rows of L (exact == 0)
+    int exact;          // 1: |H^t> decomposition, 0: |L>
+    int chunk;          // j per work item
+    long long chi;      // decomposition size
+    long long nchunks;  // work items per record
+    long long rec_off;  // first record of this instance
+    long long term_off; // first per-term output slot
+    uint64_t L[64];
+};
+
+struct SweepArgs {
+    int mode;                 // 0 sampled, 1 exact
+    int ninst;
+    long long nitems;
+    const long long *item_off;  // ninst + 1 prefix of items
+    const InstDesc *inst;
+    const uint64_t *recs;
+    unsigned long long *queue;
+    int32_t *terms;           // optional per-term output
+    long long *zsum;          // mode 0: 4 per record
+    double *kahan;            // mode 0: 2 per record; mode 1: 2 per instance
+    unsigned long long *hist; // HIST_STRIDE per instance
+    long long hist_stride;
+    unsigned long long *ips;  // per instance
+    int *err;                 // set to 2 if an inner product left [-64, 0]
+};
+
+#ifdef __HIPCC__
+namespace cse {
+
+struct Rec {
+    static DEV void store(uint64_t *rec, const AState &s, int piv, u64 Pm, int k, int n, int sexp) {
+        const int x = lane_id();
+        if (x == 0) {
+            rec[0] = (uint64_t)k | ((uint64_t)n << 8) | ((uint64_t)(s.Q & 7) << 16) | ((uint64_t)sexp << 24);
+            rec[1] = s.h;
+            rec[2] = s.act;
+            rec[3] = Pm;
+        }
+        rec[4 + x] = s.G;
+        rec[68 + x] = s.J;
+        ((uint32_t *)(rec + 132))[x] = (uint32_t)(s.d & 7) | ((uint32_t)(piv & 0xff) << 8);
+    }
+    static DEV void store_vanished(uint64_t *rec, int n) {
+        if (lane_id() == 0) {
+            rec[0] = ((uint64_t)n << 8) | (1ull << 40);
+            rec[1] = 0;
+            rec[2] = 0;
+            rec[3] = 0;
+        }
+    }
+};
+
+struct RecView {
+    u64 G, J;   // per lane
+    int d, piv; // per lane
+    int k, n, Q, sexp, vanished;
+    u64 h, act, Pm;
+    DEV void load(const uint64_t *rec) {
+        const int x = lane_id();
+        const uint64_t w0 = rec[0];
+        k = (int)(w0 & 0xff);
+        n = (int)((w0 >> 8) & 0xff);
+        Q = (int)((w0 >> 16) & 7);
+        sexp = (int)((w0 >> 24) & 0xffff);
+        vanished = (int)((w0 >> 40) & 1);
+        h = rec[1];
+        act = rec[2];
+        Pm = rec[3];
+        if (vanished) return;
+        G = rec[4 + x];
+        J = rec[68 + x];
+        const uint32_t dp = ((const uint32_t *)(rec + 132))[x];
+        d = (int)(dp & 7);
+        piv = (int)((dp >> 8) & 0xff);
+        if (piv > 63) piv = 63;
+    }
+};
+
+}  // namespace cse
+#endif

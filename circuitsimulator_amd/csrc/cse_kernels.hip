@@ -1,0 +1,574 @@
+// cse_kernels.hip -- gfx950 kernels of the stabilizer-rank inner-product
+// engine.  See cse_device.h for the execution model.
+//
+// Kernels
+//   k_ip_pairs       general <s2|s1> for packed state pairs, one wave per pair
+//   k_prep_states    prepL / prepH states (reference row bookkeeping)
+//   k_random_states  Philox random stabilizer states, projected by P
+//   k_records        theta' = Pi phi_i or Pi theta_s, turned into an
+//                    elimination-ready record (reduced row echelon basis)
+//   k_sweep          persistent: a device work queue of (record, chunk) items;
+//                    every wave computes <phi_j | theta'> for the chunk's j,
+//                    phi_j generated on the fly from (L, j) or (t, j), and
+//                    accumulates exactly (int64) plus a Kahan-FP32 shadow.
+#include "cse_device.h"
+#include "cse_internal.h"
+
+using namespace cse;
+
+namespace {
+
+// ------------------------------------------------------ packed state I/O
+DEV void load_rstate(RState &s, const uint64_t *w) {
+    const int x = lane_id();
+    uint64_t hdr = w[0];
+    s.n = (int)(hdr & 0xff);
+    s.k = (int)((hdr >> 8) & 0xff);
+    s.Q = (int)((hdr >> 16) & 7);
+    s.h = w[1];
+    u64 D1 = w[2], D2 = w[3];
+    s.d = 2 * (int)((D1 >> x) & 1ull) + 4 * (int)((D2 >> x) & 1ull);
+    const int n = s.n;
+    s.G = x < n ? w[4 + x] : 0ull;
+    s.Gb = x < n ? w[4 + n + x] : 0ull;
+    s.J = x < n ? w[4 + 2 * n + x] : 0ull;
+}
+
+DEV void store_rstate(const RState &s, uint64_t *w) {
+    const int x = lane_id();
+    const int n = s.n;
+    u64 D1 = ballot(x < n && (s.d & 2));
+    u64 D2 = ballot(x < n && (s.d & 4));
+    if (x == 0) {
+        w[0] = (uint64_t)n | ((uint64_t)s.k << 8) | ((uint64_t)(s.Q & 7) << 16);
+        w[1] = s.h;
+        w[2] = D1;
+        w[3] = D2;
+    }
+    if (x < n) {
+        w[4 + x] = s.G;
+        w[4 + n + x] = s.Gb;
+        w[4 + 2 * n + x] = s.J;
+    }
+}
+
+// General alignment of innerProductExact (stabilizer.c:611-652) in the
+// active-mask layout: the quadratic form of s2, rewritten in the basis of
+// the restricted s1 (updateQD(y) then updateDJ(R) on a copy of s2), is
+// subtracted from s1's form.
+DEV void as_subtract_general(AState &s, const RState &s2) {
+    const int x = lane_id();
+    const int n = s2.n, k2 = s2.k;
+    const u64 k2m = lowmask(k2);
+    // y_b = (h + h2) . Gbar2_b
+    const u64 y = ballot(x < n && par64((s.h ^ s2.h) & s2.Gb));
+    // updateQD(y) on the copy of s2 (stabilizer.c:163-177)
+    const bool inY = x < k2 && ((y >> x) & 1ull);
+    int Q2 = s2.Q;
+    Q2 += 2 * __popcll(ballot(inY && (s2.d & 2))) + 4 * __popcll(ballot(inY && (s2.d & 4)));
+    Q2 += 4 * (__popcll(ballot(inY && par64(s2.J & y & k2m & ~lowmask(x + 1)))) & 1);
+    Q2 &= 7;
+    int d2 = s2.d;
+    if (x < n && par64(s2.J & y)) d2 ^= 4;
+    // R_a = (G_a . Gbar2_b)_b for the active rows a (zero beyond k2)
+    u64 R = 0;
+    for (int b = 0; b < k2; b++) {
+        u64 g = rl64(s2.Gb, b);
+        R |= (u64)par64(s.G & g) << b;
+    }
+    // D2'_a = sum_{b<k2} D2_b R_ab + 4 [sum_{b>c} J2_bc R_ab R_ac odd]
+    const u64 M1 = ballot(x < k2 && (d2 & 2));
+    const u64 M2 = ballot(x < k2 && (d2 & 4));
+    int dp = 2 * __popcll(R & M1) + 4 * __popcll(R & M2);
+    u64 RJ = 0;
+    int cross = 0;
+    for (int b = 0; b < k2; b++) {
+        u64 jb = rl64(s2.J, b);
+        if ((R >> b) & 1ull) {
+            RJ ^= jb;
+            cross ^= par64(jb & R & lowmask(b));
+        }
+    }
+    dp = (dp + 4 * cross) & 7;
+    // J2' = R J2 R^T on the active rows
+    u64 J2p = 0;
+    u64 act = s.act;
+    while (act) {
+        int a2 = __ffsll((long long)act) - 1;
+        act &= act - 1;
+        u64 Ra2 = rl64(R, a2);
+        J2p |= (u64)par64(RJ & Ra2) << a2;
+    }
+    s.d = (s.d - dp + 8) & 7;
+    s.J ^= J2p;
+    s.Q = (s.Q - Q2 + 8) & 7;
+}
+
+// <s2|s1> for general packed states (stabilizer.c:589-659)
+DEV void ip_general(const RState &r1, const RState &r2, int &eps, int &p, int &m) {
+    AState s;
+    s.G = r1.G;
+    s.J = r1.J;
+    s.d = r1.d;
+    s.act = lowmask(r1.k);
+    s.h = r1.h;
+    s.Q = r1.Q;
+    for (int b = r2.k; b < r2.n; b++) {
+        u64 xi = rl64(r2.Gb, b);
+        int alpha = par64(r2.h & xi);
+        if (!as_restrict(s, xi, alpha)) {
+            eps = 0; p = 0; m = 0;
+            return;
+        }
+    }
+    as_subtract_general(s, r2);
+    int pp, mm;
+    eps = as_expsum(s, pp, mm);
+    p = pp - r1.k - r2.k;
+    m = mm;
+}
+
+}  // namespace
+
+// ======================================================================
+extern "C" __global__ void __launch_bounds__(256)
+k_ip_pairs(const uint64_t *__restrict__ s1, const uint64_t *__restrict__ s2, int n, int npairs,
+           int32_t *__restrict__ out) {
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (wave >= npairs) return;
+    const size_t stride = (size_t)(3 * n + 4);
+    RState a, b;
+    load_rstate(a, s1 + stride * wave);
+    load_rstate(b, s2 + stride * wave);
+    int eps, p, m;
+    ip_general(a, b, eps, p, m);
+    if (lane_id() == 0) {
+        out[3 * wave] = eps;
+        out[3 * wave + 1] = p;
+        out[3 * wave + 2] = m;
+    }
+}
+
+namespace {
+
+// stateprep.c:85-120 in the reference row bookkeeping
+DEV void rs_prepL(RState &s, u64 i, int t, const uint64_t *L, int k) {
+    u64 xv = 0;
+    for (int j = 0; j < k; j++)
+        if ((i >> (k - 1 - j)) & 1ull) xv ^= L[j];
+    rs_init(s, t);
+    for (int q = 0; q < t; q++)
+        if (!((xv >> q) & 1ull)) rs_shrink(s, bit(q), 0, false);
+}
+
+// stateprep.c:36-81 in the reference row bookkeeping
+DEV void rs_prepH(RState &s, u64 i, int t) {
+    const int x = lane_id();
+    const int size = (t + 1) / 2;
+    rs_init(s, t);
+    for (int j = 0; j < size; j++) {
+        int b = (int)((i >> (size - 1 - j)) & 1ull);
+        if (b == 0 && !((t & 1) && j == size - 1)) {
+            if (x == 2 * j) s.J |= bit(2 * j + 1);
+            if (x == 2 * j + 1) s.J |= bit(2 * j);
+        }
+    }
+    for (int j = 0; j < size; j++) {
+        int b = (int)((i >> (size - 1 - j)) & 1ull);
+        if ((t & 1) && j == size - 1) {
+            if (b) rs_shrink(s, bit(t - 1), 0, false);
+            continue;
+        }
+        if (b) rs_shrink(s, bit(2 * j) | bit(2 * j + 1), 0, false);
+    }
+}
+
+// projection by P (innerprod.c:98-116): number of 2^-1/2 factors, -1 if 0
+DEV int rs_project(RState &s, const DevProjector &P) {
+    int sexp = 0;
+    for (int g = 0; g < P.nstabs; g++) {
+        int r = rs_measure_pauli(s, P.phase[g] & 3, P.zs[g], P.xs[g]);
+        if (r == 0) return -1;
+        if (r == 2) sexp++;
+    }
+    return sexp;
+}
+
+// reduced row echelon basis of the active rows; pivot column per row
+DEV void to_record(const RState &r, int sexp, uint64_t *rec) {
+    const int x = lane_id();
+    AState s;
+    s.G = r.G;
+    s.J = r.J;
+    s.d = r.d;
+    s.act = lowmask(r.k);
+    s.h = r.h;
+    s.Q = r.Q;
+    int piv = 64;
+    u64 Pm = 0;
+    for (int a = 0; a < r.k; a++) {
+        u64 Ga = rl64(s.G, a);
+        int pc = __ffsll((long long)Ga) - 1;
+        if (x == a) piv = pc;
+        Pm |= bit(pc);
+        u64 Sp = ballot(x < r.k && x != a && ((s.G >> pc) & 1ull));
+        as_basis_change(s, a, Sp, true);
+    }
+    Rec::store(rec, s, piv, Pm, r.k, r.n, sexp);
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256)
+k_prep_states(int exact, int t, const uint64_t *__restrict__ L, int k, const uint64_t *__restrict__ idx,
+              int count, uint64_t *__restrict__ out) {
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (wave >= count) return;
+    RState s;
+    if (exact) rs_prepH(s, idx[wave], t);
+    else rs_prepL(s, idx[wave], t, L, k);
+    store_rstate(s, out + (size_t)(3 * t + 4) * wave);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_random_states(int n, const uint64_t *__restrict__ thr, uint64_t seed, long long first, int count,
+                DevProjector P, int use_proj, uint64_t *__restrict__ out, int32_t *__restrict__ sexp,
+                int *__restrict__ err) {
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (wave >= count) return;
+    RState s;
+    if (!rs_random_state(s, n, thr, seed, (uint64_t)(first + wave))) {
+        if (lane_id() == 0) atomicOr(err, 1);
+        return;
+    }
+    int se = 0;
+    if (use_proj) se = rs_project(s, P);
+    if (out) store_rstate(s, out + (size_t)(3 * n + 4) * wave);
+    if (sexp && lane_id() == 0) sexp[wave] = se;
+}
+
+// Records of projected states, one wave per record g (global index over all
+// instances; instance found from the record prefix).
+// mode 0: theta_s = random stabilizer state s = first + local index
+// mode 1: phi_i = prepH(i) (exact) or prepL(i) (|L>), i = local index
+// then Pi (innerprod.c:95-116 / :218-241) and the elimination-ready record.
+extern "C" __global__ void __launch_bounds__(256)
+k_records(int mode, int ninst, const long long *__restrict__ rec_off, const InstDesc *__restrict__ inst,
+          const DevProjector *__restrict__ projs, const uint64_t *__restrict__ thr_table, uint64_t seed,
+          long long first, long long nrec, uint64_t *__restrict__ recs, int32_t *__restrict__ sexp_out,
+          int *__restrict__ err) {
+    const long long g = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (g >= nrec) return;
+    int in = 0;
+    if (ninst > 1) {
+        int lo = 0, hi = ninst - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (rec_off[mid] <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        in = lo;
+    }
+    const InstDesc &D = inst[in];
+    const long long local = g - rec_off[in];
+    const int n = D.n;
+    uint64_t *rec = recs + (size_t)REC_WORDS * (size_t)g;
+    RState s;
+    if (mode == 0) {
+        if (!rs_random_state(s, n, thr_table + 65 * n, seed, (uint64_t)(first + local))) {
+            if (lane_id() == 0) {
+                atomicOr(err, 1);
+                sexp_out[g] = -1;
+            }
+            Rec::store_vanished(rec, n);
+            return;
+        }
+    } else if (D.exact) {
+        rs_prepH(s, (u64)local, n);
+    } else {
+        rs_prepL(s, (u64)local, n, D.L, D.k);
+    }
+    const int se = rs_project(s, projs[in]);
+    if (lane_id() == 0) sexp_out[g] = se;
+    if (se < 0) {
+        Rec::store_vanished(rec, n);
+        return;
+    }
+    to_record(s, se, rec);
+}
+
+// ======================================================================
+// The sweep.
+namespace {
+
+struct Acc {
+    // exact Z[w] sum of the current item at scale 2^32 (sampled mode)
+    long long c0, c1, c2, c3;
+    // Kahan FP32 shadow
+    float sr, cr, si, ci;
+    DEV void clear() {
+        c0 = c1 = c2 = c3 = 0;
+        sr = cr = si = ci = 0.f;
+    }
+    DEV void kahan(float vr, float vi) {
+        float y = vr - cr;
+        float t = sr + y;
+        cr = (t - sr) - y;
+        sr = t;
+        y = vi - ci;
+        t = si + y;
+        ci = (t - si) - y;
+        si = t;
+    }
+    // + eps 2^{p/2} w^m into the Z[w] coordinates (scale 2^32, p >= -64)
+    DEV void zomega(int p, int m) {
+        const long long v = 1ll << (32 + (p >> 1));
+        const int mm = m & 7;
+        if (!(p & 1)) {
+            const long long s = (mm & 4) ? -v : v;
+            const int q = mm & 3;
+            c0 += q == 0 ? s : 0;
+            c1 += q == 1 ? s : 0;
+            c2 += q == 2 ? s : 0;
+            c3 += q == 3 ? s : 0;
+        } else {
+            // sqrt2 w^m = w^{m+1} - w^{m+3}
+            const int q1 = (mm + 1) & 7, q2 = (mm + 3) & 7;
+            const long long s1 = (q1 & 4) ? -v : v;
+            const long long s2 = (q2 & 4) ? v : -v;
+            c0 += ((q1 & 3) == 0 ? s1 : 0) + ((q2 & 3) == 0 ? s2 : 0);
+            c1 += ((q1 & 3) == 1 ? s1 : 0) + ((q2 & 3) == 1 ? s2 : 0);
+            c2 += ((q1 & 3) == 2 ? s1 : 0) + ((q2 & 3) == 2 ? s2 : 0);
+            c3 += ((q1 & 3) == 3 ? s1 : 0) + ((q2 & 3) == 3 ? s2 : 0);
+        }
+    }
+};
+
+// FP32 value of 2^{P/2} e^{i pi m/4}
+DEV void term_f32(int P, int m, float &re, float &im) {
+    const float r = 0.70710678118654752f;
+    const int mm = m & 7;
+    const float c = (mm == 0) ? 1.f : (mm == 4) ? -1.f : (mm == 2 || mm == 6) ? 0.f : (mm == 1 || mm == 7) ? r : -r;
+    const float s = (mm == 2) ? 1.f : (mm == 6) ? -1.f : (mm == 0 || mm == 4) ? 0.f : (mm == 1 || mm == 3) ? r : -r;
+    const float mag = ldexpf((P & 1) ? 1.41421356237309505f : 1.f, P >> 1);
+    re = mag * c;
+    im = mag * s;
+}
+
+// <phi_j | theta'> for phi_j = prepL(j) (a product of |0> and |+>): the
+// coordinates outside supp(x_j) are fixed to 0.  Pivot coordinates of the
+// record become a batch of variable fixations; the rest are restrictions.
+DEV int ip_phiL(const RecView &rv, u64 xj, int n, int &p, int &m) {
+    const int x = lane_id();
+    const u64 nm = lowmask(n);
+    const u64 Z = ~xj & nm;
+    const bool act = (rv.act >> x) & 1;
+    const bool inF = act && ((Z >> rv.piv) & 1ull);
+    const u64 F = ballot(inF);
+    const bool inB = inF && ((rv.h >> rv.piv) & 1ull);
+    const u64 B = ballot(inB);
+    AState s;
+    s.G = rv.G;
+    s.J = rv.J;
+    int Q = rv.Q;
+    Q += 2 * __popcll(ballot(inB && (rv.d & 2))) + 4 * __popcll(ballot(inB && (rv.d & 4)));
+    Q += 4 * (__popcll(ballot(inB && par64(rv.J & B & lowmask(x)))) & 1);
+    s.Q = Q & 7;
+    s.d = rv.d ^ (par64(rv.J & B) << 2);
+    s.act = rv.act & ~F;
+    u64 Zn = Z & ~rv.Pm;
+    if (Zn) {
+        s.h = rv.h ^ wave_xor(inB ? rv.G : 0ull);
+        while (Zn) {
+            const int q = __ffsll((long long)Zn) - 1;
+            Zn &= Zn - 1;
+            if (!as_restrict(s, bit(q), 0)) { p = 0; m = 0; return 0; }
+        }
+    }
+    int pp, mm;
+    const int eps = as_expsum(s, pp, mm);
+    p = pp - rv.k - __popcll(xj);
+    m = mm;
+    return eps;
+}
+
+// <phi_j | theta'> for phi_j = prepH(j): pair constraints x_{2q} = x_{2q+1}
+// (bit 1), couplings 4 x_{2q} x_{2q+1} (bit 0), last qubit of odd t fixed to
+// 0 on bit 1 (stateprep.c:36-81, as a coordinate-form state).
+DEV int ip_phiH(const RecView &rv, u64 j, int t, int &p, int &m) {
+    AState s;
+    s.G = rv.G;
+    s.J = rv.J;
+    s.d = rv.d;
+    s.act = rv.act;
+    s.h = rv.h;
+    s.Q = rv.Q;
+    const int size = (t + 1) / 2;
+    int k2 = t;
+    for (int q = 0; q < size; q++) {
+        const int b = (int)((j >> (size - 1 - q)) & 1ull);
+        if (!b) continue;
+        u64 xi = ((t & 1) && q == size - 1) ? bit(t - 1) : (bit(2 * q) | bit(2 * q + 1));
+        k2--;
+        if (!as_restrict(s, xi, 0)) { p = 0; m = 0; return 0; }
+    }
+    for (int q = 0; q < size; q++) {
+        const int b = (int)((j >> (size - 1 - q)) & 1ull);
+        if (b || ((t & 1) && q == size - 1)) continue;
+        as_add_coupling(s, 2 * q, 2 * q + 1);
+    }
+    int pp, mm;
+    const int eps = as_expsum(s, pp, mm);
+    p = pp - rv.k - k2;
+    m = mm;
+    return eps;
+}
+
+DEV u64 lrow_combo(const uint64_t *L, int k, u64 j) {
+    u64 xv = 0;
+    for (int r = 0; r < k; r++)
+        if ((j >> (k - 1 - r)) & 1ull) xv ^= L[r];
+    return xv;
+}
+
+// item -> instance by binary search on the item prefix
+DEV int find_instance(const long long *item_off, int ninst, long long item) {
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (item_off[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace
+
+// Persistent sweep.  mode 0 (sampled): record r is theta_s, all j in [0, chi)
+// contribute to per-record Z[w] sums.  mode 1 (exact): record r is theta'_i
+// of its instance, j in [i, chi), weight 1 on i == j and 2 Re otherwise, into
+// the instance's (P, M) histogram.
+extern "C" __global__ void __launch_bounds__(256)
+k_sweep(SweepArgs A) {
+    __shared__ unsigned int lhist[4][LH_BINS];
+    const int wv = (int)(threadIdx.x >> 6);
+    const int x = lane_id();
+    unsigned int *hist = lhist[wv];
+    for (int b = x; b < LH_BINS; b += 64) hist[b] = 0;
+
+    while (true) {
+        long long item = 0;
+        if (x == 0) item = (long long)atomicAdd((unsigned long long *)A.queue, 1ull);
+        item = (long long)rl64((u64)item, 0);
+        if (item >= A.nitems) break;
+
+        const int inst = A.ninst > 1 ? find_instance(A.item_off, A.ninst, item) : 0;
+        const InstDesc &D = A.inst[inst];
+        const long long local = item - A.item_off[inst];
+        const long long r = local / D.nchunks;
+        const long long c = local % D.nchunks;
+        long long j0 = c * (long long)D.chunk;
+        long long j1 = j0 + D.chunk;
+        if (j1 > D.chi) j1 = D.chi;
+        if (A.mode == 1 && j0 < r) j0 = r;
+        if (j0 >= j1) continue;
+
+        const uint64_t *rec = A.recs + (size_t)REC_WORDS * (size_t)(D.rec_off + r);
+        RecView rv;
+        rv.load(rec);
+        if (rv.vanished) {
+            if (A.terms) {
+                for (long long j = j0 + x; j < j1; j += 64) {
+                    long long ti = A.mode == 0 ? (D.term_off + r * D.chi + j)
+                                               : (D.term_off + r * D.chi - r * (r - 1) / 2 + (j - r));
+                    A.terms[ti] = 0;
+                }
+            }
+            continue;
+        }
+
+        Acc acc;
+        acc.clear();
+        long long nips = 0;
+        for (long long j = j0; j < j1; j++) {
+            int p = 0, m = 0, eps;
+            if (D.exact) eps = ip_phiH(rv, (u64)j, D.n, p, m);
+            else eps = ip_phiL(rv, lrow_combo(D.L, D.k, (u64)j), D.n, p, m);
+            nips++;
+            if (A.terms && x == 0) {
+                long long ti = A.mode == 0 ? (D.term_off + r * D.chi + j)
+                                           : (D.term_off + r * D.chi - r * (r - 1) / 2 + (j - r));
+                A.terms[ti] = eps ? ((-p) * 8 + (m & 7) + 1) : 0;
+            }
+            if (!eps) continue;
+            if (p > 0 || p < -64) {
+                if (x == 0) atomicOr(A.err, 2);
+                continue;
+            }
+            const int bin = (-p) * 8 + (m & 7);
+            if (A.mode == 0) {
+                acc.zomega(p, m);
+                float vr, vi;
+                term_f32(p, m, vr, vi);
+                acc.kahan(vr, vi);
+                if (x == 0) atomicAdd(&hist[bin], 1u);
+            } else {
+                float vr, vi;
+                term_f32(p - rv.sexp, m, vr, vi);
+                if (j == r) {
+                    acc.kahan(vr, vi);
+                    if (x == 0) atomicAdd(&hist[bin], 1u);
+                } else {
+                    acc.kahan(2.f * vr, 0.f);
+                    if (x == 0) {
+                        atomicAdd(&hist[bin], 1u);
+                        atomicAdd(&hist[(-p) * 8 + ((-m) & 7)], 1u);
+                    }
+                }
+            }
+        }
+        // flush the item (this wave's LDS histogram: order its atomics first)
+        __threadfence_block();
+        if (A.mode == 0) {
+            if (x == 0) {
+                long long *z = A.zsum + 4 * (size_t)(D.rec_off + r);
+                atomicAdd((unsigned long long *)&z[0], (unsigned long long)acc.c0);
+                atomicAdd((unsigned long long *)&z[1], (unsigned long long)acc.c1);
+                atomicAdd((unsigned long long *)&z[2], (unsigned long long)acc.c2);
+                atomicAdd((unsigned long long *)&z[3], (unsigned long long)acc.c3);
+                double *kz = A.kahan + 2 * (size_t)(D.rec_off + r);
+                atomicAdd(&kz[0], (double)acc.sr - (double)acc.cr);
+                atomicAdd(&kz[1], (double)acc.si - (double)acc.ci);
+            }
+            // global (p, m) histogram of every term, shifted by nothing
+            for (int b = x; b < LH_BINS; b += 64) {
+                unsigned int v = hist[b];
+                if (v) {
+                    atomicAdd((unsigned long long *)&A.hist[(size_t)inst * A.hist_stride + b], (unsigned long long)v);
+                    hist[b] = 0;
+                }
+            }
+        } else {
+            if (x == 0) {
+                double *kz = A.kahan + 2 * (size_t)inst;
+                atomicAdd(&kz[0], (double)acc.sr - (double)acc.cr);
+                atomicAdd(&kz[1], (double)acc.si - (double)acc.ci);
+            }
+            const int shift = rv.sexp * 8;
+            for (int b = x; b < LH_BINS; b += 64) {
+                unsigned int v = hist[b];
+                if (v) {
+                    atomicAdd((unsigned long long *)&A.hist[(size_t)inst * A.hist_stride + b + shift],
+                              (unsigned long long)v);
+                    hist[b] = 0;
+                }
+            }
+        }
+        if (x == 0) atomicAdd((unsigned long long *)&A.ips[inst], (unsigned long long)nips);
+    }
+}
+
+extern "C" __global__ void k_zero64(unsigned long long *p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0ull;
+}

@@ -1,0 +1,218 @@
+"""HIP engine vs the CPU oracle on the same seeded inputs, through the C ABI
+(libcse.so).  Integer work must match bit for bit: per-term (eps, p, m mod 8)
+for every inner product, full states where the device mirrors the reference's
+row bookkeeping, exact per-sample sums and histograms."""
+import ctypes
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import orc
+from gpu_util import (from_packed, full_equal, oracle_exact_terms, oracle_sampled, pack_term,
+                      random_commuting_projector, term_value, to_packed)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def E():
+    from circuitsimulator_amd import Engine
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def _mixed_states(n, seed, count=10):
+    rng = np.random.default_rng(seed)
+    thr = orc.thresholds(n)
+    states = []
+    for st in range(count):
+        s = orc.random_state(n, seed, st, thr)
+        if st % 2:
+            ph, xs, zs = random_commuting_projector(rng, n, int(rng.integers(1, max(2, n // 2))))
+            t = s.copy()
+            if orc.project(t, orc.make_projector(ph, xs, zs, n)) >= 0:
+                s = t
+        states.append(s)
+    L = [orc.rand_bits(rng, n) for _ in range(min(6, n))]
+    states += [orc.prepL(int(rng.integers(0, 2 ** len(L))), n, L) for _ in range(3)]
+    if n <= 32:
+        states += [orc.prepH(int(rng.integers(0, 2 ** ((n + 1) // 2))), n) for _ in range(3)]
+    return states
+
+
+@pytest.mark.parametrize("n", [1, 3, 8, 24, 48, 64])
+def test_inner_products_all_pairs(E, n):
+    states = _mixed_states(n, 40 + n)
+    pk = [to_packed(s) for s in states]
+    A = np.array([pk[a] for a in range(len(pk)) for b in range(len(pk))])
+    B = np.array([pk[b] for a in range(len(pk)) for b in range(len(pk))])
+    out, _ = E.inner_products(n, A, B)
+    idx = 0
+    for a in states:
+        for b in states:
+            eps, p, m = orc.inner(a, b)
+            g = out[idx]
+            assert g[0] == eps, (n, idx)
+            if eps:
+                assert (g[1], g[2] % 8) == (p, m % 8), (n, idx)
+            idx += 1
+
+
+def test_inner_products_one_launch_per_pair(E):
+    n = 48
+    states = _mixed_states(n, 7, 6)
+    pk = [to_packed(s) for s in states]
+    A = np.array(pk[:-1])
+    B = np.array(pk[1:])
+    out1, _ = E.inner_products(n, A, B, one_launch_per_pair=True)
+    out2, _ = E.inner_products(n, A, B, one_launch_per_pair=False)
+    assert np.array_equal(out1, out2)
+    for i in range(len(states) - 1):
+        eps, p, m = orc.inner(states[i], states[i + 1])
+        assert out1[i][0] == eps and (not eps or (out1[i][1], out1[i][2] % 8) == (p, m % 8))
+
+
+@pytest.mark.parametrize("t", [1, 2, 5, 6, 11, 24, 48, 64])
+def test_prep_states_full_state(E, t):
+    rng = np.random.default_rng(t)
+    k = min(t, 10)
+    L = [orc.rand_bits(rng, t) for _ in range(k)]
+    idx = [int(v) for v in rng.integers(0, 2 ** k, size=8)]
+    got = E.prep_states(0, t, L, idx)
+    for c, i in enumerate(idx):
+        assert full_equal(from_packed(got[c]), orc.prepL(i, t, L)), (t, i)
+    if t <= 40:
+        idx = [int(v) for v in rng.integers(0, 2 ** ((t + 1) // 2), size=8)]
+        got = E.prep_states(1, t, [], idx)
+        for c, i in enumerate(idx):
+            assert full_equal(from_packed(got[c]), orc.prepH(i, t)), (t, i)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 24, 48, 64])
+def test_random_states_full_state(E, n):
+    seed = 0xC0FFEE + n
+    got, sexp = E.random_states(n, seed, 100, 24)
+    thr = orc.thresholds(n)
+    for c in range(24):
+        want = orc.random_state(n, seed, 100 + c, thr)
+        assert full_equal(from_packed(got[c]), want), (n, c)
+
+
+@pytest.mark.parametrize("n,r", [(6, 3), (24, 12), (48, 24)])
+def test_random_states_projected_full_state(E, n, r):
+    from circuitsimulator_amd.engine import Projector
+    rng = np.random.default_rng(n)
+    ph, xs, zs = random_commuting_projector(rng, n, r)
+    seed = 77 + n
+    got, sexp = E.random_states(n, seed, 0, 20, Projector.make(ph, xs, zs, n))
+    thr = orc.thresholds(n)
+    Po = orc.make_projector(ph, xs, zs, n)
+    for c in range(20):
+        want = orc.random_state(n, seed, c, thr)
+        e = orc.project(want, Po)
+        assert sexp[c] == e, (n, c)
+        if e >= 0:
+            assert full_equal(from_packed(got[c]), want), (n, c)
+
+
+def _exact_mean(t, sexp, zsum):
+    """exact sampled estimator as a Fraction-free float via integers: sum of
+    2^{t-s-64} (A + B sqrt2) / count, evaluated with Python integers"""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    tot = Decimal(0)
+    r2 = Decimal(2).sqrt()
+    for s, c in zip(sexp, zsum):
+        if s < 0:
+            continue
+        c0, c1, c2, c3 = (int(v) for v in c)
+        A = c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3
+        B = c0 * c1 - c0 * c3 + c1 * c2 + c2 * c3
+        tot += (Decimal(A) + Decimal(B) * r2) * (Decimal(2) ** (t - int(s) - 64))
+    return float(tot / len(sexp))
+
+
+@pytest.mark.parametrize("t,r,k,exact", [(6, 3, 3, 0), (8, 4, 4, 0), (7, 2, 0, 1), (10, 5, 5, 0), (12, 6, 0, 1)])
+def test_sampled_small_all_terms(E, t, r, k, exact):
+    from circuitsimulator_amd.engine import Projector
+    rng = np.random.default_rng(100 * t + r)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    L = [orc.rand_bits(rng, t) for _ in range(k)]
+    chi = (1 << ((t + 1) // 2)) if exact else (1 << k)
+    seed, first, count = 1234 + t, 50, 48
+    res, ex = E.sampled_projector(Projector.make(ph, xs, zs, t), L, k, exact, seed, first, count,
+                                  chunk=max(1, chi // 3), want_terms=True)
+    se, zsum, terms = oracle_sampled(t, k, L, exact, ph, xs, zs, seed, first, count, chi)
+    assert np.array_equal(ex["sexp"], se)
+    assert np.array_equal(ex["terms"], terms)
+    assert np.array_equal(ex["zsum"], zsum)
+    assert res.ips == int((se >= 0).sum()) * chi
+    want = _exact_mean(t, se, zsum)
+    assert abs(res.mean - want) <= 1e-15 * max(1.0, abs(want))
+    # FP32-Kahan shadow: relative error bound of FP32 accumulation
+    assert abs(res.mean_kahan - want) <= 1e-5 * max(1.0, abs(want))
+
+
+@pytest.mark.parametrize("t,r,exact,k", [(1, 1, 1, 0), (3, 2, 1, 0), (6, 3, 1, 0), (5, 2, 0, 3), (9, 4, 0, 4),
+                                         (8, 8, 1, 0)])
+def test_exact_all_terms(E, t, r, exact, k):
+    from circuitsimulator_amd.engine import Instance, Projector
+    rng = np.random.default_rng(7 * t + r + 100 * exact)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    L = [orc.rand_bits(rng, t) for _ in range(k)]
+    inst = Instance()
+    inst.P = Projector.make(ph, xs, zs, t)
+    inst.exact = exact
+    inst.k = k
+    for i, v in enumerate(L):
+        inst.L[i] = v
+    chi = (1 << ((t + 1) // 2)) if exact else (1 << k)
+    nl = chi * (chi + 1) // 2
+    res, terms, _ = E.batch_exact([inst], chunk=max(1, chi // 2), want_terms=True, nterms=nl)
+    want = oracle_exact_terms(t, k, L, exact, ph, xs, zs)
+    assert [w[0] for w in want] == list(terms)
+    tot = 0
+    for (pt, s, i, j) in want:
+        if pt:
+            v = term_value(pt, s)
+            tot += v if i == j else 2 * v.real
+    assert abs(res[0].re - tot.real) <= 1e-12 * max(1, abs(tot))
+    assert res[0].im == 0.0
+    assert abs(res[0].value - abs(tot)) <= 1e-12 * max(1, abs(tot))
+    assert abs(res[0].kahan_re - tot.real) <= 1e-5 * max(1, abs(tot))
+
+
+def test_batch_uneven_instances(E):
+    from circuitsimulator_amd.engine import Instance, Projector
+    rng = np.random.default_rng(2024)
+    insts, specs = [], []
+    for _ in range(12):
+        t = int(rng.integers(2, 13))
+        r = int(rng.integers(1, t // 2 + 1))
+        exact = int(rng.integers(0, 2)) if t <= 10 else 0
+        k = 0 if exact else int(rng.integers(1, min(t, 5) + 1))
+        ph, xs, zs = random_commuting_projector(rng, t, r)
+        L = [orc.rand_bits(rng, t) for _ in range(k)]
+        inst = Instance()
+        inst.P = Projector.make(ph, xs, zs, t)
+        inst.exact = exact
+        inst.k = k
+        for i, v in enumerate(L):
+            inst.L[i] = v
+        insts.append(inst)
+        specs.append((t, k, L, exact, ph, xs, zs))
+    nterms = 0
+    for (t, k, L, exact, *_r) in specs:
+        chi = (1 << ((t + 1) // 2)) if exact else (1 << k)
+        nterms += chi * (chi + 1) // 2
+    res, terms, _ = E.batch_exact(insts, chunk=7, want_terms=True, nterms=nterms)
+    off = 0
+    for q, (t, k, L, exact, ph, xs, zs) in enumerate(specs):
+        want = oracle_exact_terms(t, k, L, exact, ph, xs, zs)
+        got = list(terms[off:off + len(want)])
+        assert got == [w[0] for w in want], q
+        off += len(want)
+        assert res[q].im == 0.0
