@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the stabilizer-rank inner-product engine on one MI355X.
+
+Metric: inner products/s at t=48 (batched persistent engine vs the plain
+one-launch-per-pair kernel), with |error| of the sampled estimator against the
+FP64 oracle.  A step is one full norm-estimation pass (innerprod.c:45-144):
+M random stabilizer states theta drawn on the device, each projected by the
+r=24 Pauli group G, then <phi_i|theta'> for all chi=2^12 decomposition states
+phi_i of |L> -- M*chi = 2^28 inner products per step at the default M=2^16.
+
+Printed: one JSON line (rank 0).  `value` is the whole-call throughput
+(theta generation + projection + sweep + exact estimator), inputs resident.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# VALU lane-operations per inner product of k_sweep on this workload:
+# SQ_INSTS_VALU (wave instructions per dispatch, rocprofv3 --pmc) x 64 lanes
+# / inner products per dispatch.  Source: profiles/k_sweep_t48_c4_pmc.md.
+VALU_LANE_OPS_PER_IP = None
+# HBM bytes per k_sweep dispatch at M=2^16 (FETCH_SIZE + WRITE_SIZE,
+# calibrated against a copy kernel).  Source: profiles/k_sweep_t48_c4_pmc.md.
+SWEEP_TRAFFIC_BYTES = None
+PEAK_INT_TOPS = 39.3  # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz, int32 lane-ops/s
+
+
+def exact_mean(t, sexp, zsum):
+    """FP64 oracle of the estimator: the exact value sum_s 2^{t-s-64}(A + B sqrt2) / M,
+    from the exact per-sample Z[omega] sums, rounded once."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 80
+    r2 = Decimal(2).sqrt()
+    tot = Decimal(0)
+    for s, c in zip(sexp.tolist(), zsum.tolist()):
+        if s < 0:
+            continue
+        c0, c1, c2, c3 = c
+        A = c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3
+        B = c0 * c1 - c0 * c3 + c1 * c2 + c2 * c3
+        tot += (Decimal(A) + Decimal(B) * r2) * (Decimal(2) ** (t - s - 64))
+    return float(tot / len(sexp))
+
+
+def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0):
+    """The reference's own C (oracle/_ref/libref.so, compiled from the reference
+    sources) on a bounded sample of the same workload: the first nphi
+    decomposition states for `threads` projected thetas, one POSIX thread per
+    theta.  Falls back to the oracle (a port) when libref.so is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes
+
+    import orc
+    ph, xs, zs = P_tuple
+    Po = orc.make_projector(ph, xs, zs, t)
+    thr = orc.thresholds(t)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    thetas = []
+    st = 0
+    while len(thetas) < threads:
+        th = orc.random_state(t, seed, st, thr)
+        st += 1
+        if orc.project(th, Po) >= 0:
+            thetas.append(th)
+    Larr = (orc.U64 * k)(*L)
+    if orc.ref_available():
+        R = orc.ref()
+        arr1 = (orc.OrcState * 1)(thetas[0])
+        z = (orc.I64 * 4)()
+        t0 = time.perf_counter()
+        R.ref_sampled_sweep(arr1, 1, t, Larr, k, 32, 1, 2, z)
+        per_ip = (time.perf_counter() - t0) / 2
+        nphi = int(max(2, min(1 << k, budget_s / per_ip)))
+        arr = (orc.OrcState * threads)(*thetas)
+        z = (orc.I64 * (4 * threads))()
+        t0 = time.perf_counter()
+        n = R.ref_sampled_sweep(arr, threads, t, Larr, k, 32, threads, nphi, z)
+        dt = time.perf_counter() - t0
+        return {"value": n / dt, "unit": "inner products/s", "cores": threads, "kind": "reference",
+                "sample": f"{threads} projected thetas x first {nphi} of {1 << k} |L> states at t={t} "
+                          f"({n} inner products, reference stabilizer.c/stateprep.c built -O2, "
+                          f"{threads} threads, {dt:.1f} s)"}
+    # oracle port, single thread
+    phis = [orc.prepL(i, t, L) for i in range(64)]
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < budget_s:
+        for p in phis:
+            orc.inner(thetas[n % len(thetas)], p)
+            n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "inner products/s", "cores": 1, "kind": "port",
+            "sample": f"{n} inner products theta' x prepL at t={t} by the oracle, 1 thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--samples", type=int, default=65536, help="random thetas per step (M)")
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--plain-pairs", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this engine is measured on one GPU; multi-GPU runs are not part of this build "
+              "(the per-GPU sharding design is described in the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from circuitsimulator_amd import Engine
+    from circuitsimulator_amd.workloads import sampled_config
+
+    t, r, k = 48, 24, 12
+    M = args.samples
+    P, P_tuple, L = sampled_config(t, r, k, seed=1004)
+    chi = 1 << k
+    with Engine(0) as E:
+        cus, devname = E.device_info()
+        for w in range(args.warmup):
+            E.sampled_projector(P, L, k, 0, 1004 + 100 + w, 0, M, chunk=args.chunk, want_samples=False)
+        ips = 0
+        kernel_ms = prep_ms = 0.0
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            last = s == args.steps - 1
+            res, ex = E.sampled_projector(P, L, k, 0, 1004 + s, 0, M, chunk=args.chunk, want_samples=last)
+            ips += res.ips
+            kernel_ms += res.kernel_ms
+            prep_ms += res.prep_ms
+        dt = time.perf_counter() - t0
+        value = ips / dt
+        kernel_ips = ips / (kernel_ms / 1000.0)
+
+        # precision study on the last step's thetas
+        oracle = exact_mean(t, ex["sexp"], ex["zsum"])
+        err_int64 = abs(res.mean - oracle)
+        err_kahan = abs(res.mean_kahan - oracle)
+
+        # plain: the general per-pair kernel, one launch per pair
+        npairs = args.plain_pairs
+        th, sexp = E.random_states(t, 7, 0, 64, P)
+        ok = np.nonzero(sexp >= 0)[0]
+        rng = np.random.default_rng(5)
+        A = th[ok[rng.integers(0, len(ok), size=npairs)]]
+        B = E.prep_states(0, t, L, rng.integers(0, chi, size=npairs))
+        E.inner_products(t, A[:64], B[:64], one_launch_per_pair=True)  # warm
+        _, plain_ms = E.inner_products(t, A, B, one_launch_per_pair=True)
+        _, general_batched_ms = E.inner_products(t, A, B, one_launch_per_pair=False)
+        plain_ips = npairs / (plain_ms / 1000.0)
+
+    out = {
+        "metric": "inner products/s, batched vs plain, t=48 on 1 GPU; |err| vs FP64 oracle",
+        "value": value,
+        "unit": "inner products/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1000.0 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded Philox random stabilizer states; seeded commuting Pauli group and L)",
+        "config": {"workload": f"norm estimation t={t} r={r} k={k} (chi={chi}) M={M} random thetas, "
+                               f"{M * chi} inner products per step",
+                   "t": t, "r": r, "k": k, "chi": chi, "samples": M, "seed": 1004},
+        "device": devname,
+        "kernel_ip_per_s": kernel_ips,
+        "sweep_ms_per_step": kernel_ms / args.steps,
+        "prep_ms_per_step": prep_ms / args.steps,
+        "plain_ip_per_s": plain_ips,
+        "plain_pairs": npairs,
+        "general_batched_ip_per_s": npairs / (general_batched_ms / 1000.0),
+        "batched_vs_plain": value / plain_ips,
+        "estimator": {"int64_exact": res.mean, "kahan_fp32": res.mean_kahan, "fp64_oracle": oracle,
+                      "abs_err_int64": err_int64, "abs_err_kahan_fp32": err_kahan},
+    }
+    if VALU_LANE_OPS_PER_IP:
+        achieved = kernel_ips * VALU_LANE_OPS_PER_IP / 1e12
+        out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_TOPS, "unit": "Tops/s",
+                           "frac": achieved / PEAK_INT_TOPS, "traffic": SWEEP_TRAFFIC_BYTES}
+    else:
+        out["roofline"] = None
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(P_tuple, L, t, k, 1004)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -24,6 +24,7 @@
 
 #define DEV __device__ __forceinline__
 
+
 typedef unsigned long long u64;
 
 namespace cse {
@@ -263,76 +264,120 @@ DEV bool rs_random_state(RState &s, int n, const uint64_t *thr, uint64_t key, ui
 }
 
 // ---------------------------------------------------- active-mask layout
+// Per lane x: G row x and J row x (off-diagonal bits only are meaningful: J
+// stays symmetric, so bit x of row i is J_xi) and D_x in Z8 (always even).
+// Uniform: the active rows, h and Q.
+//
+// Packed form (PK = true, n <= 61): D_x lives in bits 61..63 of the J row,
+// so one broadcast of row a gives both the J-row and D_a, and adding to D is
+// a 64-bit add on the row (mod 8 in the top three bits, no carry into the
+// row).  Unpacked form (PK = false): D_x in its own register.
+#define CSE_ROWMASK 0x1FFFFFFFFFFFFFFFull
+
 struct AState {
-    u64 G, J;   // per lane
-    int d;      // per lane, Z8
-    u64 act;    // uniform: active rows
-    u64 h;      // uniform
-    int Q;      // uniform
+    u64 G, J;
+    int d;      // PK = false only
+    u64 act;
+    u64 h;
+    int Q;
 };
 
-// Basis change R = I + sum_{b in Sp} e_b e_i^T (updateDJ of the reference,
-// stabilizer.c:129-160, for a rank-1 R; G_b ^= G_i when withG).  The J
-// diagonal is not trusted: J_ii is taken from D_i.  Returns (through rjxi)
-// bit i of each row of R J, which the fixation needs.
-DEV int as_basis_change(AState &s, int i, u64 Sp, bool withG) {
-    const int x = lane_id();
-    const bool inSp = (Sp >> x) & 1;
-    u64 Ji = rl64(s.J, i);
-    const int di = rl32(s.d, i);
-    const u64 jii = (u64)((di >> 1) & 1);
-    Ji = (Ji & ~bit(i)) | (jii << i);
-    u64 Gi = 0;
-    if (withG) Gi = rl64(s.G, i);
-    const int jxi = (int)((s.J >> i) & 1ull);
-    if (inSp) {
-        s.d = (s.d + di + 4 * jxi) & 7;
-        s.J ^= Ji;
-        if (withG) s.G ^= Gi;
+// per-lane predicate "lane in mask" for a wave-uniform mask (no compare)
+DEV bool lane_in(u64 mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
+
+// all-ones if b else 0 (uniform integer select)
+DEV u64 allmask(int b) { return 0ull - (u64)(b & 1); }
+
+template <bool PK> DEV int d_of_row(u64 row, const AState &s, int a) {
+    if (PK) return (int)(row >> 61);
+    return rl32(s.d, a);
+}
+template <bool PK> DEV u64 row_bits(u64 row) { return PK ? (row & CSE_ROWMASK) : row; }
+// D_x += v on lanes in U (v uniform, 0..7)
+template <bool PK> DEV void d_add(AState &s, u64 U, int v) {
+    if (PK) {
+        if (lane_in(U)) s.J += (u64)v << 61;
+    } else {
+        if (lane_in(U)) s.d = (s.d + v) & 7;
     }
-    const int rjxi = (int)((s.J >> i) & 1ull);
-    if (rjxi) s.J ^= Sp;
-    return rjxi;
+}
+// D_x += 4 on lanes in U
+template <bool PK> DEV void d_add4(AState &s, u64 U) {
+    if (PK) {
+        if (lane_in(U)) s.J ^= bit(63);
+    } else {
+        if (lane_in(U)) s.d ^= 4;
+    }
+}
+
+// Basis change R = I + sum_{b in Sp} e_b e_i^T of the quadratic form
+// (updateDJ of the reference, stabilizer.c:129-160, for a rank-1 R):
+//   D_b += D_i + 4 J_bi, J <- R J R^T  (and G_b ^= G_i when withG).
+// J_ii is taken as [D_i in {2,6}].  Returns M = column i of R J (bit x =
+// (RJ)_xi), which fixing z_i needs; *di_out = D_i.
+template <bool PK> DEV u64 as_basis_change(AState &s, int i, u64 Sp, bool withG, int *di_out) {
+    const u64 bi = bit(i);
+    const u64 Jr = rl64(s.J, i);
+    const int di = d_of_row<PK>(Jr, s, i);
+    const int jii = (di >> 1) & 1;
+    const u64 Jc = (row_bits<PK>(Jr) & ~bi) | ((u64)jii << i);
+    const u64 M = Jc ^ (Sp & allmask(jii));
+    if (withG) {
+        const u64 Gi = rl64(s.G, i);
+        if (lane_in(Sp)) s.G ^= Gi;
+    }
+    // rows b in Sp: J_b ^= Jc, D_b += D_i + 4 J_bi (J_bi = bit b of Jc)
+    if (PK) {
+        if (lane_in(Sp)) s.J = (s.J ^ Jc) + ((u64)di << 61);
+        if (lane_in(Sp & Jc)) s.J ^= bit(63);
+    } else {
+        const int four = lane_in(Jc) ? 4 : 0;
+        if (lane_in(Sp)) {
+            s.d = (s.d + di + four) & 7;
+            s.J ^= Jc;
+        }
+    }
+    // columns: rows x with (RJ)_xi = 1 get J_x ^= Sp
+    if (lane_in(M)) s.J ^= Sp;
+    *di_out = di;
+    return M;
 }
 
 // Basis change as above, then fix z_i = c and retire row i.
-DEV void as_eliminate(AState &s, int i, u64 Sp, int c, bool withG) {
-    const int di = rl32(s.d, i);
-    const int rjxi = as_basis_change(s, i, Sp, withG);
+template <bool PK> DEV void as_eliminate(AState &s, int i, u64 Sp, int c, bool withG) {
+    int di;
+    const u64 M = as_basis_change<PK>(s, i, Sp, withG, &di);
     if (c) {
         s.Q = (s.Q + di) & 7;
-        s.d = (s.d + 4 * rjxi) & 7;
+        d_add4<PK>(s, M);
         if (withG) s.h ^= rl64(s.G, i);
     }
     s.act &= ~bit(i);
 }
 
 // Restrict to {x : x . xi = alpha}.  Returns false if the restriction is empty.
-DEV bool as_restrict(AState &s, u64 xi, int alpha) {
-    const int x = lane_id();
-    u64 S = ballot(((s.act >> x) & 1) && par64(s.G & xi));
+template <bool PK> DEV bool as_restrict(AState &s, u64 xi, int alpha) {
+    u64 S = ballot(lane_in(s.act) && par64(s.G & xi));
     int beta = (alpha ^ __popcll(s.h & xi)) & 1;
     if (S == 0) return beta == 0;
     int i = 63 - __clzll(S);
-    as_eliminate(s, i, S & ~bit(i), beta, true);
+    as_eliminate<PK>(s, i, S & ~bit(i), beta, true);
     return true;
 }
 
 // Adds 4 x_u x_v (as a function of the affine coordinates) to the form:
 // the coupling of an |00>+|01>+|10>-|11> pair (stateprep.c:45-50).
-DEV void as_add_coupling(AState &s, int u, int v) {
-    const int x = lane_id();
-    const bool a = (s.act >> x) & 1;
+template <bool PK> DEV void as_add_coupling(AState &s, int u, int v) {
     const int gu = (int)((s.G >> u) & 1ull), gv = (int)((s.G >> v) & 1ull);
-    u64 cu = ballot(a && gu), cv = ballot(a && gv);
+    const u64 cu = ballot(lane_in(s.act) && gu), cv = ballot(lane_in(s.act) && gv);
     const int hu = (int)((s.h >> u) & 1ull), hv = (int)((s.h >> v) & 1ull);
     s.Q = (s.Q + 4 * (hu & hv)) & 7;
-    if (a) {
-        int lin = (hu & gv) ^ (hv & gu) ^ (gu & gv);
-        s.d = (s.d + 4 * lin) & 7;
-        u64 t = (gu ? cv : 0ull) ^ (gv ? cu : 0ull);
-        s.J ^= t;
-    }
+    // linear part: 4 z_a (hu g_va + hv g_ua + g_ua g_va)
+    const u64 lin = (cv & allmask(hu)) ^ (cu & allmask(hv)) ^ (cu & cv);
+    d_add4<PK>(s, lin);
+    // quadratic part: J_ab ^= g_ua g_vb + g_ub g_va
+    if (lane_in(cu)) s.J ^= cv;
+    if (lane_in(cv)) s.J ^= cu;
 }
 
 // sum_{z in F2^act} w^{q(z)} by eliminating one variable at a time:
@@ -341,37 +386,48 @@ DEV void as_add_coupling(AState &s, int u, int v) {
 //  D_a in {0,4}: sum over z_a gives 2 [l = D_a/4]; a non-empty J-row makes
 //    that a linear constraint, solved by as_eliminate on its last member.
 // Returns eps; p is the exponent of sqrt2, m the phase (mod 8, includes Q).
-DEV int as_expsum(AState &s, int &p, int &m) {
-    const int x = lane_id();
-    int pp = 0, macc = 0;
-    while (s.act) {
-        const int a = __ffsll((long long)s.act) - 1;
+template <bool PK> DEV int as_expsum(AState &s, int &p, int &m) {
+    int pp = 0, macc = 0, zero = 0;
+    u64 act = s.act;
+    while (act) {
+        const int a = __ffsll((long long)act) - 1;
+        act &= ~bit(a);
         const u64 Ja = rl64(s.J, a);
-        const int da = rl32(s.d, a);
-        s.act &= ~bit(a);
-        const u64 eta = Ja & s.act;
+        const u64 eta = Ja & act;
+        const int da = d_of_row<PK>(Ja, s, a);
+        const int h4 = (da >> 2) & 1;
         if (da & 2) {
-            const int sg = (da == 2) ? 1 : 7;  // +1 or -1 (mod 8)
+            // D_a = 2 (s = +1) or 6 (s = -1): factor sqrt2 w^s; D_b -= 2s
+            // and J_bc ^= 1 on the J-row eta of a
             pp += 1;
-            macc += sg;
-            if ((eta >> x) & 1) {
-                s.d = (s.d - 2 * ((da == 2) ? 1 : -1) + 8) & 7;
+            macc += h4 ? 7 : 1;
+            const int delta = h4 ? 2 : 6;
+            if (PK) {
+                if (lane_in(eta)) s.J = (s.J ^ eta) + ((u64)delta << 61);
+            } else if (lane_in(eta)) {
+                s.d = (s.d + delta) & 7;
                 s.J ^= eta;
             }
-        } else {
+        } else if (eta) {
+            // factor 2 and the constraint sum_{b in eta} z_b = D_a/4
             pp += 2;
-            const int c = da >> 2;
-            if (eta == 0) {
-                if (c) { p = 0; m = 0; return 0; }
-            } else {
-                const int i = 63 - __clzll(eta);
-                as_eliminate(s, i, eta & ~bit(i), c, false);
-            }
+            const int i = 63 - __clzll(eta);
+            act &= ~bit(i);
+            int di;
+            const u64 M = as_basis_change<PK>(s, i, eta & ~bit(i), false, &di);
+            s.Q = (s.Q + (di & -h4)) & 7;
+            d_add4<PK>(s, M & allmask(h4));
+        } else {
+            // factor 1 + w^{D_a}: 2 if D_a = 0, the whole sum vanishes if 4
+            pp += 2;
+            zero |= h4;
+            act &= (u64)h4 - 1ull;
         }
     }
-    p = pp;
-    m = (s.Q + macc) & 7;
-    return 1;
+    s.act = 0;
+    p = zero ? 0 : pp;
+    m = zero ? 0 : ((s.Q + macc) & 7);
+    return zero ? 0 : 1;
 }
 
 }  // namespace cse

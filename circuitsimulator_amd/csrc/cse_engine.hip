@@ -223,7 +223,10 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     A.ips = dp<unsigned long long>(ctx->ips);
     A.err = dp<int>(ctx->errflag);
     const int grid = sweep_grid(ctx, pl.nitems);
-    hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(256), 0, ctx->stream, A);
+    bool packed = true;
+    for (const InstDesc &d : pl.inst) packed = packed && d.n <= 61;
+    if (packed) hipLaunchKernelGGL(k_sweep_packed, dim3(grid), dim3(256), 0, ctx->stream, A);
+    else hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(256), 0, ctx->stream, A);
     HIPCHK(hipGetLastError());
     return CSE_OK;
 }

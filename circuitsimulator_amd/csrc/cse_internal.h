@@ -15,14 +15,15 @@ struct DevProjector {
     uint64_t zs[64];
 };
 
-// Elimination-ready record of a projected state theta' (reduced row echelon
-// basis of its active rows), REC_WORDS uint64 words:
+// Elimination-ready record of a projected state theta', REC_WORDS uint64
+// words.  The basis of the active rows is in reduced row echelon form and row
+// a is stored at index p_a, its pivot column, so that the active-row mask is
+// the set of pivot columns and x_a = h_a + z_a for every active a.
 //   w[0] = k | n << 8 | Q << 16 | sexp << 24 | vanished << 40
-//   w[1] = h, w[2] = active-row mask, w[3] = mask of pivot columns
-//   w[4 .. 68)     G rows
-//   w[68 .. 132)   J rows
-//   w[132 .. 164)  64 x uint32: D_x | pivot column << 8
-#define REC_WORDS 164
+//   w[1] = h, w[2] = active rows (= pivot columns), w[3] = D1, w[4] = D2
+//   w[5 .. 69)     G rows
+//   w[69 .. 133)   J rows (symmetric; the diagonal is not used)
+#define REC_WORDS 133
 
 // Histogram bins per instance: P in [-(64 + 64), 0], 8 phases.
 #define HIST_STRIDE (129 * 8)
@@ -62,33 +63,22 @@ struct SweepArgs {
 namespace cse {
 
 struct Rec {
-    static DEV void store(uint64_t *rec, const AState &s, int piv, u64 Pm, int k, int n, int sexp) {
-        const int x = lane_id();
-        if (x == 0) {
-            rec[0] = (uint64_t)k | ((uint64_t)n << 8) | ((uint64_t)(s.Q & 7) << 16) | ((uint64_t)sexp << 24);
-            rec[1] = s.h;
-            rec[2] = s.act;
-            rec[3] = Pm;
-        }
-        rec[4 + x] = s.G;
-        rec[68 + x] = s.J;
-        ((uint32_t *)(rec + 132))[x] = (uint32_t)(s.d & 7) | ((uint32_t)(piv & 0xff) << 8);
-    }
     static DEV void store_vanished(uint64_t *rec, int n) {
         if (lane_id() == 0) {
             rec[0] = ((uint64_t)n << 8) | (1ull << 40);
             rec[1] = 0;
             rec[2] = 0;
             rec[3] = 0;
+            rec[4] = 0;
         }
     }
 };
 
 struct RecView {
     u64 G, J;   // per lane
-    int d, piv; // per lane
+    int d;      // per lane: D_x
     int k, n, Q, sexp, vanished;
-    u64 h, act, Pm;
+    u64 h, act, D1, D2;
     DEV void load(const uint64_t *rec) {
         const int x = lane_id();
         const uint64_t w0 = rec[0];
@@ -99,14 +89,12 @@ struct RecView {
         vanished = (int)((w0 >> 40) & 1);
         h = rec[1];
         act = rec[2];
-        Pm = rec[3];
+        D1 = rec[3];
+        D2 = rec[4];
         if (vanished) return;
-        G = rec[4 + x];
-        J = rec[68 + x];
-        const uint32_t dp = ((const uint32_t *)(rec + 132))[x];
-        d = (int)(dp & 7);
-        piv = (int)((dp >> 8) & 0xff);
-        if (piv > 63) piv = 63;
+        G = rec[5 + x];
+        J = rec[69 + x];
+        d = 2 * (int)((D1 >> x) & 1ull) + 4 * (int)((D2 >> x) & 1ull);
     }
 };
 

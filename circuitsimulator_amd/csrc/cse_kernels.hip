@@ -6,7 +6,8 @@
 //   k_prep_states    prepL / prepH states (reference row bookkeeping)
 //   k_random_states  Philox random stabilizer states, projected by P
 //   k_records        theta' = Pi phi_i or Pi theta_s, turned into an
-//                    elimination-ready record (reduced row echelon basis)
+//                    elimination-ready record (reduced row echelon basis,
+//                    rows indexed by pivot column)
 //   k_sweep          persistent: a device work queue of (record, chunk) items;
 //                    every wave computes <phi_j | theta'> for the chunk's j,
 //                    phi_j generated on the fly from (L, j) or (t, j), and
@@ -50,6 +51,16 @@ DEV void store_rstate(const RState &s, uint64_t *w) {
         w[4 + n + x] = s.Gb;
         w[4 + 2 * n + x] = s.J;
     }
+}
+
+DEV void to_astate(const RState &r, AState &s) {
+    const int x = lane_id();
+    s.G = r.G;
+    s.J = r.J;
+    s.d = x < r.n ? (r.d & 6) : 0;
+    s.act = lowmask(r.k);
+    s.h = r.h;
+    s.Q = r.Q;
 }
 
 // General alignment of innerProductExact (stabilizer.c:611-652) in the
@@ -99,6 +110,7 @@ DEV void as_subtract_general(AState &s, const RState &s2) {
         u64 Ra2 = rl64(R, a2);
         J2p |= (u64)par64(RJ & Ra2) << a2;
     }
+    // D -= D2', J ^= J2', Q -= Q2'
     s.d = (s.d - dp + 8) & 7;
     s.J ^= J2p;
     s.Q = (s.Q - Q2 + 8) & 7;
@@ -107,24 +119,19 @@ DEV void as_subtract_general(AState &s, const RState &s2) {
 // <s2|s1> for general packed states (stabilizer.c:589-659)
 DEV void ip_general(const RState &r1, const RState &r2, int &eps, int &p, int &m) {
     AState s;
-    s.G = r1.G;
-    s.J = r1.J;
-    s.d = r1.d;
-    s.act = lowmask(r1.k);
-    s.h = r1.h;
-    s.Q = r1.Q;
+    to_astate(r1, s);
     for (int b = r2.k; b < r2.n; b++) {
         u64 xi = rl64(r2.Gb, b);
         int alpha = par64(r2.h & xi);
-        if (!as_restrict(s, xi, alpha)) {
+        if (!as_restrict<false>(s, xi, alpha)) {
             eps = 0; p = 0; m = 0;
             return;
         }
     }
     as_subtract_general(s, r2);
     int pp, mm;
-    eps = as_expsum(s, pp, mm);
-    p = pp - r1.k - r2.k;
+    eps = as_expsum<false>(s, pp, mm);
+    p = eps ? pp - r1.k - r2.k : 0;
     m = mm;
 }
 
@@ -194,27 +201,56 @@ DEV int rs_project(RState &s, const DevProjector &P) {
     return sexp;
 }
 
-// reduced row echelon basis of the active rows; pivot column per row
+DEV u64 wave_or(u64 v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v |= shfl_xor64(v, off);
+    return uniform64(v);
+}
+
+// Reduced row echelon basis of the active rows (basis changes of the form,
+// so the state is unchanged), then rows and columns of J and the D planes
+// re-indexed so that row a sits at its pivot column.
 DEV void to_record(const RState &r, int sexp, uint64_t *rec) {
     const int x = lane_id();
     AState s;
-    s.G = r.G;
-    s.J = r.J;
-    s.d = r.d;
-    s.act = lowmask(r.k);
-    s.h = r.h;
-    s.Q = r.Q;
-    int piv = 64;
+    to_astate(r, s);
+    int piv = 0;
     u64 Pm = 0;
     for (int a = 0; a < r.k; a++) {
-        u64 Ga = rl64(s.G, a);
-        int pc = __ffsll((long long)Ga) - 1;
+        const u64 Ga = rl64(s.G, a);
+        const int pc = __ffsll((long long)Ga) - 1;
         if (x == a) piv = pc;
         Pm |= bit(pc);
-        u64 Sp = ballot(x < r.k && x != a && ((s.G >> pc) & 1ull));
-        as_basis_change(s, a, Sp, true);
+        const u64 Sp = ballot(x < r.k && x != a && ((s.G >> pc) & 1ull));
+        int dummy;
+        as_basis_change<false>(s, a, Sp, true, &dummy);
     }
-    Rec::store(rec, s, piv, Pm, r.k, r.n, sexp);
+    // D planes: bit a -> bit piv_a
+    const bool act = x < r.k;
+    const u64 D1 = wave_or((act && (s.d & 2)) ? bit(piv) : 0ull);
+    const u64 D2 = wave_or((act && (s.d & 4)) ? bit(piv) : 0ull);
+    // J columns: bit b -> bit piv_b (for active b); rows scattered below
+    u64 Jp = 0;
+    for (int b = 0; b < r.k; b++) {
+        const int pb = rl32(piv, b);
+        Jp |= ((s.J >> b) & 1ull) << pb;
+    }
+    if (x == 0) {
+        rec[0] = (uint64_t)r.k | ((uint64_t)r.n << 8) | ((uint64_t)(s.Q & 7) << 16) | ((uint64_t)sexp << 24);
+        rec[1] = s.h;
+        rec[2] = Pm;
+        rec[3] = D1;
+        rec[4] = D2;
+    }
+    // rows of non-pivot columns are never active: clear them, then scatter
+    if (!((Pm >> x) & 1ull)) {
+        rec[5 + x] = 0;
+        rec[69 + x] = 0;
+    }
+    if (act) {
+        rec[5 + piv] = s.G;
+        rec[69 + piv] = Jp;
+    }
 }
 
 }  // namespace
@@ -301,93 +337,41 @@ k_records(int mode, int ninst, const long long *__restrict__ rec_off, const Inst
 // The sweep.
 namespace {
 
-struct Acc {
-    // exact Z[w] sum of the current item at scale 2^32 (sampled mode)
-    long long c0, c1, c2, c3;
-    // Kahan FP32 shadow
-    float sr, cr, si, ci;
-    DEV void clear() {
-        c0 = c1 = c2 = c3 = 0;
-        sr = cr = si = ci = 0.f;
-    }
-    DEV void kahan(float vr, float vi) {
-        float y = vr - cr;
-        float t = sr + y;
-        cr = (t - sr) - y;
-        sr = t;
-        y = vi - ci;
-        t = si + y;
-        ci = (t - si) - y;
-        si = t;
-    }
-    // + eps 2^{p/2} w^m into the Z[w] coordinates (scale 2^32, p >= -64)
-    DEV void zomega(int p, int m) {
-        const long long v = 1ll << (32 + (p >> 1));
-        const int mm = m & 7;
-        if (!(p & 1)) {
-            const long long s = (mm & 4) ? -v : v;
-            const int q = mm & 3;
-            c0 += q == 0 ? s : 0;
-            c1 += q == 1 ? s : 0;
-            c2 += q == 2 ? s : 0;
-            c3 += q == 3 ? s : 0;
-        } else {
-            // sqrt2 w^m = w^{m+1} - w^{m+3}
-            const int q1 = (mm + 1) & 7, q2 = (mm + 3) & 7;
-            const long long s1 = (q1 & 4) ? -v : v;
-            const long long s2 = (q2 & 4) ? v : -v;
-            c0 += ((q1 & 3) == 0 ? s1 : 0) + ((q2 & 3) == 0 ? s2 : 0);
-            c1 += ((q1 & 3) == 1 ? s1 : 0) + ((q2 & 3) == 1 ? s2 : 0);
-            c2 += ((q1 & 3) == 2 ? s1 : 0) + ((q2 & 3) == 2 ? s2 : 0);
-            c3 += ((q1 & 3) == 3 ? s1 : 0) + ((q2 & 3) == 3 ? s2 : 0);
-        }
-    }
-};
-
-// FP32 value of 2^{P/2} e^{i pi m/4}
-DEV void term_f32(int P, int m, float &re, float &im) {
-    const float r = 0.70710678118654752f;
-    const int mm = m & 7;
-    const float c = (mm == 0) ? 1.f : (mm == 4) ? -1.f : (mm == 2 || mm == 6) ? 0.f : (mm == 1 || mm == 7) ? r : -r;
-    const float s = (mm == 2) ? 1.f : (mm == 6) ? -1.f : (mm == 0 || mm == 4) ? 0.f : (mm == 1 || mm == 3) ? r : -r;
-    const float mag = ldexpf((P & 1) ? 1.41421356237309505f : 1.f, P >> 1);
-    re = mag * c;
-    im = mag * s;
-}
-
 // <phi_j | theta'> for phi_j = prepL(j) (a product of |0> and |+>): the
-// coordinates outside supp(x_j) are fixed to 0.  Pivot coordinates of the
-// record become a batch of variable fixations; the rest are restrictions.
-DEV int ip_phiL(const RecView &rv, u64 xj, int n, int &p, int &m) {
-    const int x = lane_id();
-    const u64 nm = lowmask(n);
-    const u64 Z = ~xj & nm;
-    const bool act = (rv.act >> x) & 1;
-    const bool inF = act && ((Z >> rv.piv) & 1ull);
-    const u64 F = ballot(inF);
-    const bool inB = inF && ((rv.h >> rv.piv) & 1ull);
-    const u64 B = ballot(inB);
+// coordinates outside supp(x_j) are fixed to 0.  On pivot columns this fixes
+// the variable itself (x_a = h_a + z_a), all at once; the remaining zero
+// coordinates are restrictions.
+template <bool PK>
+DEV int ip_phiL(const RecView &rv, u64 lowx, u64 xj, int n, int &p, int &m) {
+    const u64 Z = ~xj & lowmask(n);
+    const u64 F = Z & rv.act;
+    const u64 B = F & rv.h;
     AState s;
-    s.G = rv.G;
-    s.J = rv.J;
-    int Q = rv.Q;
-    Q += 2 * __popcll(ballot(inB && (rv.d & 2))) + 4 * __popcll(ballot(inB && (rv.d & 4)));
-    Q += 4 * (__popcll(ballot(inB && par64(rv.J & B & lowmask(x)))) & 1);
+    // Q += sum_{a in B} D_a + 4 #{edges of J inside B}; D_x += 4 |J_x & B|
+    int Q = rv.Q + 2 * __popcll(rv.D1 & B) + 4 * __popcll(rv.D2 & B);
+    const u64 JB = rv.J & B;
+    Q += 4 * (__popcll(ballot(lane_in(B) && par64(JB & lowx))) & 1);
     s.Q = Q & 7;
-    s.d = rv.d ^ (par64(rv.J & B) << 2);
+    if (PK) {
+        s.J = rv.J ^ ((u64)par64(JB) << 63);
+    } else {
+        s.J = rv.J;
+        s.d = rv.d ^ (par64(JB) << 2);
+    }
     s.act = rv.act & ~F;
-    u64 Zn = Z & ~rv.Pm;
+    u64 Zn = Z & ~rv.act;
     if (Zn) {
-        s.h = rv.h ^ wave_xor(inB ? rv.G : 0ull);
-        while (Zn) {
+        s.G = rv.G;
+        s.h = rv.h ^ wave_xor(lane_in(B) ? rv.G : 0ull);
+        do {
             const int q = __ffsll((long long)Zn) - 1;
             Zn &= Zn - 1;
-            if (!as_restrict(s, bit(q), 0)) { p = 0; m = 0; return 0; }
-        }
+            if (!as_restrict<PK>(s, bit(q), 0)) { p = 0; m = 0; return 0; }
+        } while (Zn);
     }
     int pp, mm;
-    const int eps = as_expsum(s, pp, mm);
-    p = pp - rv.k - __popcll(xj);
+    const int eps = as_expsum<PK>(s, pp, mm);
+    p = eps ? pp - rv.k - __popcll(xj) : 0;
     m = mm;
     return eps;
 }
@@ -395,11 +379,12 @@ DEV int ip_phiL(const RecView &rv, u64 xj, int n, int &p, int &m) {
 // <phi_j | theta'> for phi_j = prepH(j): pair constraints x_{2q} = x_{2q+1}
 // (bit 1), couplings 4 x_{2q} x_{2q+1} (bit 0), last qubit of odd t fixed to
 // 0 on bit 1 (stateprep.c:36-81, as a coordinate-form state).
+template <bool PK>
 DEV int ip_phiH(const RecView &rv, u64 j, int t, int &p, int &m) {
     AState s;
     s.G = rv.G;
     s.J = rv.J;
-    s.d = rv.d;
+    if (!PK) s.d = rv.d;
     s.act = rv.act;
     s.h = rv.h;
     s.Q = rv.Q;
@@ -410,16 +395,16 @@ DEV int ip_phiH(const RecView &rv, u64 j, int t, int &p, int &m) {
         if (!b) continue;
         u64 xi = ((t & 1) && q == size - 1) ? bit(t - 1) : (bit(2 * q) | bit(2 * q + 1));
         k2--;
-        if (!as_restrict(s, xi, 0)) { p = 0; m = 0; return 0; }
+        if (!as_restrict<PK>(s, xi, 0)) { p = 0; m = 0; return 0; }
     }
     for (int q = 0; q < size; q++) {
         const int b = (int)((j >> (size - 1 - q)) & 1ull);
         if (b || ((t & 1) && q == size - 1)) continue;
-        as_add_coupling(s, 2 * q, 2 * q + 1);
+        as_add_coupling<PK>(s, 2 * q, 2 * q + 1);
     }
     int pp, mm;
-    const int eps = as_expsum(s, pp, mm);
-    p = pp - rv.k - k2;
+    const int eps = as_expsum<PK>(s, pp, mm);
+    p = eps ? pp - rv.k - k2 : 0;
     m = mm;
     return eps;
 }
@@ -442,19 +427,114 @@ DEV int find_instance(const long long *item_off, int ninst, long long item) {
     return lo;
 }
 
+// FP32 value of 2^{P/2} e^{i pi m/4}
+DEV void term_f32(int P, int m, float &re, float &im) {
+    const float r = 0.70710678118654752f;
+    const int mm = m & 7;
+    const float c = (mm == 0) ? 1.f : (mm == 4) ? -1.f : (mm == 2 || mm == 6) ? 0.f : (mm == 1 || mm == 7) ? r : -r;
+    const float s = (mm == 2) ? 1.f : (mm == 6) ? -1.f : (mm == 0 || mm == 4) ? 0.f : (mm == 1 || mm == 3) ? r : -r;
+    const float mag = ldexpf((P & 1) ? 1.41421356237309505f : 1.f, P >> 1);
+    re = mag * c;
+    im = mag * s;
+}
+
+// Per-lane accumulators: terms are parked one per lane (lane = term count
+// mod 64) and folded 64 at a time by all lanes in parallel.
+struct LaneAcc {
+    long long c0, c1, c2, c3;   // exact Z[w] partial sums, scale 2^32 (mode 0)
+    float sr, cr, si, ci;       // Kahan FP32 partial sums
+    DEV void clear() {
+        c0 = c1 = c2 = c3 = 0;
+        sr = cr = si = ci = 0.f;
+    }
+    DEV void kahan(float vr, float vi) {
+        float y = vr - cr;
+        float t = sr + y;
+        cr = (t - sr) - y;
+        sr = t;
+        y = vi - ci;
+        t = si + y;
+        ci = (t - si) - y;
+        si = t;
+    }
+    // + 2^{p/2} w^m in Z[w] coordinates (scale 2^32, -64 <= p <= 0)
+    DEV void zomega(int p, int m) {
+        const long long v = 1ll << (32 + (p >> 1));
+        const int mm = m & 7;
+        int qa, qb;
+        long long sa, sb;
+        if (!(p & 1)) {
+            qa = mm; sa = v; qb = 0; sb = 0;
+        } else {
+            // sqrt2 w^m = w^{m+1} - w^{m+3}
+            qa = (mm + 1) & 7; sa = v;
+            qb = (mm + 3) & 7; sb = -v;
+        }
+        if (qa & 4) sa = -sa;
+        if (qb & 4) sb = -sb;
+        qa &= 3;
+        qb &= 3;
+        c0 += (qa == 0 ? sa : 0) + (qb == 0 ? sb : 0);
+        c1 += (qa == 1 ? sa : 0) + (qb == 1 ? sb : 0);
+        c2 += (qa == 2 ? sa : 0) + (qb == 2 ? sb : 0);
+        c3 += (qa == 3 ? sa : 0) + (qb == 3 ? sb : 0);
+    }
+};
+
+DEV long long wave_sum64(long long v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += (long long)shfl_xor64((u64)v, off);
+    return (long long)uniform64((u64)v);
+}
+
+DEV double wave_sumf64(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Fold the parked terms (lanes < cnt).  Packed term: 0 for eps = 0, else
+// 1 + ((-p) * 8 + m) * 2 + offdiag.
+DEV void fold_terms(int packed, int cnt, int mode, int sexp, LaneAcc &acc, unsigned int *hist) {
+    const int x = lane_id();
+    if (x >= cnt || packed == 0) return;
+    const int v = packed - 1;
+    const int off = v & 1;
+    const int pm = v >> 1;
+    const int p = -(pm >> 3), m = pm & 7;
+    float vr, vi;
+    if (mode == 0) {
+        acc.zomega(p, m);
+        term_f32(p, m, vr, vi);
+        acc.kahan(vr, vi);
+        atomicAdd(&hist[pm], 1u);
+    } else {
+        term_f32(p - sexp, m, vr, vi);
+        if (off) {
+            acc.kahan(2.f * vr, 0.f);
+            atomicAdd(&hist[pm], 1u);
+            atomicAdd(&hist[(pm & ~7) | ((8 - m) & 7)], 1u);
+        } else {
+            acc.kahan(vr, vi);
+            atomicAdd(&hist[pm], 1u);
+        }
+    }
+}
+
 }  // namespace
 
 // Persistent sweep.  mode 0 (sampled): record r is theta_s, all j in [0, chi)
 // contribute to per-record Z[w] sums.  mode 1 (exact): record r is theta'_i
 // of its instance, j in [i, chi), weight 1 on i == j and 2 Re otherwise, into
 // the instance's (P, M) histogram.
-extern "C" __global__ void __launch_bounds__(256)
-k_sweep(SweepArgs A) {
+template <bool PK>
+DEV void sweep_body(const SweepArgs &A) {
     __shared__ unsigned int lhist[4][LH_BINS];
     const int wv = (int)(threadIdx.x >> 6);
     const int x = lane_id();
     unsigned int *hist = lhist[wv];
     for (int b = x; b < LH_BINS; b += 64) hist[b] = 0;
+    const u64 lowx = lowmask(x);
 
     while (true) {
         long long item = 0;
@@ -472,101 +552,101 @@ k_sweep(SweepArgs A) {
         if (j1 > D.chi) j1 = D.chi;
         if (A.mode == 1 && j0 < r) j0 = r;
         if (j0 >= j1) continue;
+        // first per-term output slot of j0
+        const long long tbase = A.mode == 0 ? (D.term_off + r * D.chi + j0)
+                                            : (D.term_off + r * D.chi - r * (r - 1) / 2 + (j0 - r));
 
         const uint64_t *rec = A.recs + (size_t)REC_WORDS * (size_t)(D.rec_off + r);
         RecView rv;
         rv.load(rec);
+        if (PK && !rv.vanished) rv.J |= (u64)rv.d << 61;
         if (rv.vanished) {
-            if (A.terms) {
-                for (long long j = j0 + x; j < j1; j += 64) {
-                    long long ti = A.mode == 0 ? (D.term_off + r * D.chi + j)
-                                               : (D.term_off + r * D.chi - r * (r - 1) / 2 + (j - r));
-                    A.terms[ti] = 0;
-                }
-            }
+            if (A.terms)
+                for (long long j = j0 + x; j < j1; j += 64) A.terms[tbase + (j - j0)] = 0;
             continue;
         }
 
-        Acc acc;
+        LaneAcc acc;
         acc.clear();
-        long long nips = 0;
+        int parked = 0, cnt = 0;
+        long long tslot = tbase;
+        // x_j of |L> incrementally: going from j-1 to j flips bits 0..c of j,
+        // c = ctz(j), i.e. rows k-1 .. k-1-c of L; lane c holds their XOR
+        u64 xj = D.exact ? 0ull : lrow_combo(D.L, D.k, (u64)j0);
+        u64 flip = 0;
+        if (!D.exact && x < D.k) {
+            for (int b = 0; b <= x; b++) flip ^= D.L[D.k - 1 - b];
+        }
         for (long long j = j0; j < j1; j++) {
             int p = 0, m = 0, eps;
-            if (D.exact) eps = ip_phiH(rv, (u64)j, D.n, p, m);
-            else eps = ip_phiL(rv, lrow_combo(D.L, D.k, (u64)j), D.n, p, m);
-            nips++;
-            if (A.terms && x == 0) {
-                long long ti = A.mode == 0 ? (D.term_off + r * D.chi + j)
-                                           : (D.term_off + r * D.chi - r * (r - 1) / 2 + (j - r));
-                A.terms[ti] = eps ? ((-p) * 8 + (m & 7) + 1) : 0;
-            }
-            if (!eps) continue;
-            if (p > 0 || p < -64) {
-                if (x == 0) atomicOr(A.err, 2);
-                continue;
-            }
-            const int bin = (-p) * 8 + (m & 7);
-            if (A.mode == 0) {
-                acc.zomega(p, m);
-                float vr, vi;
-                term_f32(p, m, vr, vi);
-                acc.kahan(vr, vi);
-                if (x == 0) atomicAdd(&hist[bin], 1u);
-            } else {
-                float vr, vi;
-                term_f32(p - rv.sexp, m, vr, vi);
-                if (j == r) {
-                    acc.kahan(vr, vi);
-                    if (x == 0) atomicAdd(&hist[bin], 1u);
+            if (j > j0) xj ^= rl64(flip, __ffsll(j) - 1);
+            if (D.exact) eps = ip_phiH<PK>(rv, (u64)j, D.n, p, m);
+            else eps = ip_phiL<PK>(rv, lowx, xj, D.n, p, m);
+            int packed = 0;
+            if (eps) {
+                if (p > 0 || p < -64) {
+                    if (x == 0) atomicOr(A.err, 2);
                 } else {
-                    acc.kahan(2.f * vr, 0.f);
-                    if (x == 0) {
-                        atomicAdd(&hist[bin], 1u);
-                        atomicAdd(&hist[(-p) * 8 + ((-m) & 7)], 1u);
-                    }
+                    packed = 1 + ((-p) * 8 + (m & 7)) * 2 + ((A.mode == 1 && j != r) ? 1 : 0);
                 }
             }
+            if (lane_in(bit(cnt))) parked = packed;
+            if (++cnt == 64) {
+                if (A.terms) {
+                    const int v = parked - 1;
+                    A.terms[tslot + x] = parked ? ((v >> 1) + 1) : 0;
+                }
+                fold_terms(parked, 64, A.mode, rv.sexp, acc, hist);
+                tslot += 64;
+                cnt = 0;
+            }
         }
-        // flush the item (this wave's LDS histogram: order its atomics first)
+        if (cnt) {
+            if (A.terms && x < cnt) {
+                const int v = parked - 1;
+                A.terms[tslot + x] = parked ? ((v >> 1) + 1) : 0;
+            }
+            fold_terms(parked, cnt, A.mode, rv.sexp, acc, hist);
+        }
+        // flush the item (the LDS histogram atomics of this wave come first)
         __threadfence_block();
+        const double kr = wave_sumf64((double)acc.sr - (double)acc.cr);
+        const double ki = wave_sumf64((double)acc.si - (double)acc.ci);
         if (A.mode == 0) {
+            const long long z0 = wave_sum64(acc.c0), z1 = wave_sum64(acc.c1);
+            const long long z2 = wave_sum64(acc.c2), z3 = wave_sum64(acc.c3);
             if (x == 0) {
                 long long *z = A.zsum + 4 * (size_t)(D.rec_off + r);
-                atomicAdd((unsigned long long *)&z[0], (unsigned long long)acc.c0);
-                atomicAdd((unsigned long long *)&z[1], (unsigned long long)acc.c1);
-                atomicAdd((unsigned long long *)&z[2], (unsigned long long)acc.c2);
-                atomicAdd((unsigned long long *)&z[3], (unsigned long long)acc.c3);
+                atomicAdd((unsigned long long *)&z[0], (unsigned long long)z0);
+                atomicAdd((unsigned long long *)&z[1], (unsigned long long)z1);
+                atomicAdd((unsigned long long *)&z[2], (unsigned long long)z2);
+                atomicAdd((unsigned long long *)&z[3], (unsigned long long)z3);
                 double *kz = A.kahan + 2 * (size_t)(D.rec_off + r);
-                atomicAdd(&kz[0], (double)acc.sr - (double)acc.cr);
-                atomicAdd(&kz[1], (double)acc.si - (double)acc.ci);
+                atomicAdd(&kz[0], kr);
+                atomicAdd(&kz[1], ki);
             }
-            // global (p, m) histogram of every term, shifted by nothing
-            for (int b = x; b < LH_BINS; b += 64) {
-                unsigned int v = hist[b];
-                if (v) {
-                    atomicAdd((unsigned long long *)&A.hist[(size_t)inst * A.hist_stride + b], (unsigned long long)v);
-                    hist[b] = 0;
-                }
-            }
-        } else {
-            if (x == 0) {
-                double *kz = A.kahan + 2 * (size_t)inst;
-                atomicAdd(&kz[0], (double)acc.sr - (double)acc.cr);
-                atomicAdd(&kz[1], (double)acc.si - (double)acc.ci);
-            }
-            const int shift = rv.sexp * 8;
-            for (int b = x; b < LH_BINS; b += 64) {
-                unsigned int v = hist[b];
-                if (v) {
-                    atomicAdd((unsigned long long *)&A.hist[(size_t)inst * A.hist_stride + b + shift],
-                              (unsigned long long)v);
-                    hist[b] = 0;
-                }
+        } else if (x == 0) {
+            double *kz = A.kahan + 2 * (size_t)inst;
+            atomicAdd(&kz[0], kr);
+            atomicAdd(&kz[1], ki);
+        }
+        const int shift = A.mode == 1 ? rv.sexp * 8 : 0;
+        for (int b = x; b < LH_BINS; b += 64) {
+            unsigned int v = hist[b];
+            if (v) {
+                atomicAdd((unsigned long long *)&A.hist[(size_t)inst * A.hist_stride + b + shift],
+                          (unsigned long long)v);
+                hist[b] = 0;
             }
         }
-        if (x == 0) atomicAdd((unsigned long long *)&A.ips[inst], (unsigned long long)nips);
+        if (x == 0) atomicAdd((unsigned long long *)&A.ips[inst], (unsigned long long)(j1 - j0));
     }
 }
+
+extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_body<false>(A); }
+
+// the same sweep with D packed into the J rows (every instance has n <= 61)
+extern "C" __global__ void __launch_bounds__(256) k_sweep_packed(SweepArgs A) { sweep_body<true>(A); }
 
 extern "C" __global__ void k_zero64(unsigned long long *p, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcse.so")
+LIB_PATH = os.environ.get("CSE_LIBCSE", os.path.join(HERE, "libcse.so"))
 BACKEND_PATH = os.path.join(HERE, "cse_backend")
 
 NMAX = 64

@@ -128,7 +128,7 @@ void ref_prepL(orc_state *s, int i, int t, const uint64_t *L, int k)
 
 /*
  * Timed CPU baseline of the sampled-mode inner loop (innerprod.c:119-139):
- * for each theta in [theta0, theta0 + ntheta) and each i < 2^k,
+ * for each of the ntheta thetas and each i < nphi (<= 2^k),
  * innerProduct(theta', prepL(i)) with the reference's own functions, with
  * nthreads POSIX threads splitting the theta range cyclically like the
  * reference's rank-strided loop (probability.c:268).  The projected thetas
@@ -137,11 +137,13 @@ void ref_prepL(orc_state *s, int i, int t, const uint64_t *L, int k)
  */
 typedef struct {
     const orc_state *thetas;
-    int ntheta, t, k, nthreads, tid, E;
+    int ntheta, t, k, nthreads, tid, E, nphi;
     const uint64_t *L;
     int64_t *csum;
     long count;
 } ref_job;
+
+static void ref_zomega_add(int64_t c[4], int E, int eps, int p, int m);
 
 static void *ref_worker(void *arg)
 {
@@ -150,11 +152,11 @@ static void *ref_worker(void *arg)
     for (int th = j->tid; th < j->ntheta; th += j->nthreads) {
         struct StabilizerState *a = to_ref(&j->thetas[th]);
         int64_t c[4] = {0, 0, 0, 0};
-        for (int i = 0; i < (1 << j->k); i++) {
+        for (int i = 0; i < j->nphi; i++) {
             struct StabilizerState *phi = prepL(i, j->t, M);
             int eps, p, m;
             innerProductExact(a, phi, &eps, &p, &m);
-            orc_zomega_add(c, j->E, eps, p, m);
+            ref_zomega_add(c, j->E, eps, p, m);
             freeStabilizerState(phi);
             j->count++;
         }
@@ -165,8 +167,8 @@ static void *ref_worker(void *arg)
     return NULL;
 }
 
-/* orc_zomega_add is the oracle's; re-implemented here so libref.so stands alone */
-void orc_zomega_add(int64_t c[4], int E, int eps, int p, int m)
+/* Z[w] accumulation (the oracle's orc_zomega_add), kept local so libref.so stands alone */
+static void ref_zomega_add(int64_t c[4], int E, int eps, int p, int m)
 {
     if (!eps) return;
     int e = E + (p >> 1);
@@ -184,14 +186,15 @@ void orc_zomega_add(int64_t c[4], int E, int eps, int p, int m)
 }
 
 long ref_sampled_sweep(const orc_state *thetas, int ntheta, int t, const uint64_t *L, int k,
-                       int E, int nthreads, int64_t *csum)
+                       int E, int nthreads, int nphi, int64_t *csum)
 {
+    if (nphi > (1 << k) || nphi < 0) nphi = 1 << k;
     if (nthreads < 1) nthreads = 1;
     pthread_t th[256];
     ref_job jobs[256];
     if (nthreads > 256) nthreads = 256;
     for (int i = 0; i < nthreads; i++) {
-        jobs[i] = (ref_job){thetas, ntheta, t, k, nthreads, i, E, L, csum, 0};
+        jobs[i] = (ref_job){thetas, ntheta, t, k, nthreads, i, E, nphi, L, csum, 0};
         pthread_create(&th[i], NULL, ref_worker, &jobs[i]);
     }
     long total = 0;

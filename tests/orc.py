@@ -123,7 +123,7 @@ def ref():
         R.ref_prepH.argtypes = [P(OrcState), ctypes.c_int, ctypes.c_int]
         R.ref_prepL.argtypes = [P(OrcState), ctypes.c_int, ctypes.c_int, P(U64), ctypes.c_int]
         R.ref_sampled_sweep.argtypes = [P(OrcState), ctypes.c_int, ctypes.c_int, P(U64), ctypes.c_int,
-                                        ctypes.c_int, ctypes.c_int, P(I64)]
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, P(I64)]
         R.ref_sampled_sweep.restype = ctypes.c_long
         _ref = R
     return _ref

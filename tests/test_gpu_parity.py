@@ -216,3 +216,67 @@ def test_batch_uneven_instances(E):
         assert got == [w[0] for w in want], q
         off += len(want)
         assert res[q].im == 0.0
+
+
+def test_sampled_t64_unpacked_layout(E):
+    # n > 61 runs the sweep with D in its own register
+    from circuitsimulator_amd.engine import Projector
+    t, r, k = 64, 4, 3
+    rng = np.random.default_rng(64)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    L = [orc.rand_bits(rng, t) for _ in range(k)]
+    res, ex = E.sampled_projector(Projector.make(ph, xs, zs, t), L, k, 0, 3, 0, 10, want_terms=True)
+    se, zsum, terms = oracle_sampled(t, k, L, 0, ph, xs, zs, 3, 0, 10, 1 << k)
+    assert np.array_equal(ex["sexp"], se)
+    assert np.array_equal(ex["terms"], terms)
+    assert np.array_equal(ex["zsum"], zsum)
+
+
+def test_sampled_t48_workload_subsample(E):
+    # the benchmark's workload shape: t=48, r=24, k=12 (chi=4096)
+    from circuitsimulator_amd.workloads import sampled_config
+    P, (ph, xs, zs), L = sampled_config(48, 24, 12, seed=1004)
+    t, k, chi, count = 48, 12, 4096, 3
+    res, ex = E.sampled_projector(P, L, k, 0, 77, 5, count, want_terms=True)
+    Po = orc.make_projector(ph, xs, zs, t)
+    thr = orc.thresholds(t)
+    rng = np.random.default_rng(48)
+    for s in range(count):
+        th = orc.random_state(t, 77, 5 + s, thr)
+        e = orc.project(th, Po)
+        assert ex["sexp"][s] == e
+        if e < 0:
+            assert not ex["terms"][s].any()
+            continue
+        for i in rng.integers(0, chi, size=120):
+            eps, p, m = orc.inner(th, orc.prepL(int(i), t, L))
+            assert ex["terms"][s][i] == pack_term(eps, p, m), (s, int(i))
+    assert res.ips == int((ex["sexp"] >= 0).sum()) * chi
+
+
+def test_exact_t24_workload_subsample(E):
+    # exact mode at t=24, r=12, k=11: chi(chi+1)/2 = 2,098,176 pair terms
+    from circuitsimulator_amd.engine import Instance
+    from circuitsimulator_amd.workloads import commuting_paulis, random_L
+    rng = np.random.default_rng(1002)
+    t, r, k = 24, 12, 11
+    ph, xs, zs = commuting_paulis(rng, t, r)
+    L = random_L(rng, t, k)
+    from circuitsimulator_amd.engine import Projector
+    inst = Instance()
+    inst.P = Projector.make(ph, xs, zs, t)
+    inst.exact = 0
+    inst.k = k
+    for i, v in enumerate(L):
+        inst.L[i] = v
+    chi = 1 << k
+    nl = chi * (chi + 1) // 2
+    res, terms, _ = E.batch_exact([inst], want_terms=True, nterms=nl)
+    assert res[0].terms == nl
+    ls = [int(v) for v in np.random.default_rng(5).integers(0, nl, size=300)]
+    want = oracle_exact_terms(t, k, L, 0, ph, xs, zs, ls=ls)
+    for l, w in zip(ls, want):
+        assert terms[l] == w[0], l
+    # the total is real: conjugate pairs and real diagonal terms
+    assert res[0].im == 0.0 and res[0].re >= 0.0
+    assert abs(res[0].kahan_re - res[0].re) <= 1e-5 * max(1.0, res[0].re)
