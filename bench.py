@@ -20,13 +20,16 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# VALU lane-operations per inner product of k_sweep on this workload:
-# SQ_INSTS_VALU (wave instructions per dispatch, rocprofv3 --pmc) x 64 lanes
-# / inner products per dispatch.  Source: profiles/k_sweep_t48_c4_pmc.md.
-VALU_LANE_OPS_PER_IP = None
-# HBM bytes per k_sweep dispatch at M=2^16 (FETCH_SIZE + WRITE_SIZE,
-# calibrated against a copy kernel).  Source: profiles/k_sweep_t48_c4_pmc.md.
-SWEEP_TRAFFIC_BYTES = None
+# Per inner product of the sweep kernel on this workload, from rocprofv3
+# --pmc over one dispatch of 2^28 inner products
+# (profiles/k_sweep_norm_estimation_t48_r24_k12_M65536.md):
+VALU_INSTS_PER_IP = 304.5    # SQ_INSTS_VALU / inner products (wave64 instructions)
+SALU_INSTS_PER_IP = 522.0    # SQ_INSTS_SALU / inner products
+VALU_LANE_OPS_PER_IP = VALU_INSTS_PER_IP * 64
+# HBM bytes per inner product of the sweep: (FETCH_SIZE x 2.000 + WRITE_SIZE x
+# 1.000) KiB, factors from a 1 GiB copy kernel, / inner products
+SWEEP_BYTES_PER_IP = 3.64
+CLOCK_GHZ = 2.39             # GRBM_GUI_ACTIVE / kernel time / 8 XCDs
 PEAK_INT_TOPS = 39.3  # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz, int32 lane-ops/s
 
 
@@ -184,12 +187,21 @@ def main():
         "estimator": {"int64_exact": res.mean, "kahan_fp32": res.mean_kahan, "fp64_oracle": oracle,
                       "abs_err_int64": err_int64, "abs_err_kahan_fp32": err_kahan},
     }
-    if VALU_LANE_OPS_PER_IP:
-        achieved = kernel_ips * VALU_LANE_OPS_PER_IP / 1e12
-        out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_TOPS, "unit": "Tops/s",
-                           "frac": achieved / PEAK_INT_TOPS, "traffic": SWEEP_TRAFFIC_BYTES}
-    else:
-        out["roofline"] = None
+    # integer-VALU roofline of the sweep kernel (int32 lane-operations/s)
+    achieved = kernel_ips * VALU_LANE_OPS_PER_IP / 1e12
+    out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_TOPS, "unit": "Tops/s",
+                       "frac": achieved / PEAK_INT_TOPS,
+                       "traffic": SWEEP_BYTES_PER_IP * M * chi,
+                       "kernel": "k_sweep_packed",
+                       "per_unit": f"{VALU_LANE_OPS_PER_IP:.0f} VALU lane-ops per inner product (measured)"}
+    # the binding limit: instruction issue (one SALU per CU per cycle, one
+    # wave64 VALU per SIMD per 4 cycles)
+    out["issue"] = {
+        "salu_frac": kernel_ips * SALU_INSTS_PER_IP / (cus * CLOCK_GHZ * 1e9),
+        "valu_frac": kernel_ips * VALU_INSTS_PER_IP / (cus * 4 * CLOCK_GHZ * 1e9 / 4),
+        "salu_per_ip": SALU_INSTS_PER_IP, "valu_per_ip": VALU_INSTS_PER_IP,
+        "hbm_GBps": kernel_ips * SWEEP_BYTES_PER_IP / 1e9,
+    }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P_tuple, L, t, k, 1004)
     print(json.dumps(out), flush=True)
