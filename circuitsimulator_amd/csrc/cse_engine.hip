@@ -246,12 +246,18 @@ InstDesc make_inst(int t, int k, int exact, const uint64_t *L, int chunk) {
     return d;
 }
 
+// chi <= 2^30 keeps every per-theta Z[omega] sum exact in int64: a term is at
+// most 2^32 in each component at the 2^32 fixed-point scale, so chi terms stay
+// below 2^62.  (The reference holds chi in an int, innerprod.c:122-131.)
+constexpr int kMaxLogChi = 30;
+
 int check_decomp(cse_ctx *ctx, int t, int k, int exact, const uint64_t *L) {
     if (exact) {
-        if ((t + 1) / 2 > 40) return fail(ctx, CSE_ERR_ARG, "exact decomposition 2^%d too large", (t + 1) / 2);
+        if ((t + 1) / 2 > kMaxLogChi)
+            return fail(ctx, CSE_ERR_ARG, "exact decomposition chi=2^%d exceeds 2^%d", (t + 1) / 2, kMaxLogChi);
         return CSE_OK;
     }
-    if (k < 0 || k > 40) return fail(ctx, CSE_ERR_ARG, "k=%d outside [0, 40]", k);
+    if (k < 0 || k > kMaxLogChi) return fail(ctx, CSE_ERR_ARG, "k=%d outside [0, %d]", k, kMaxLogChi);
     if (k > 0 && !L) return fail(ctx, CSE_ERR_ARG, "L is NULL");
     uint64_t m = t == 64 ? ~0ull : ((1ull << t) - 1);
     for (int r = 0; r < k; r++)

@@ -107,7 +107,8 @@ typedef struct {
  * replaces exactProjector / exactProjectorWork (innerprod.c:148-261) for
  * t >= 1 and a non-empty P (the empty and t = 0 cases are closed forms the
  * caller handles).  exact != 0: |H^t> decomposition (chi = 2^ceil(t/2));
- * else |L> from L (k rows, chi = 2^k).
+ * else |L> from L (k rows, chi = 2^k).  chi <= 2^30 (CSE_ERR_ARG otherwise),
+ * the range in which every sum stays exact in int64.
  */
 int cse_exact_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k,
                         int exact, cse_exact_result *res);
