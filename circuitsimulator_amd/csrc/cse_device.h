@@ -386,7 +386,175 @@ template <bool PK> DEV void as_add_coupling(AState &s, int u, int v) {
 //  D_a in {0,4}: sum over z_a gives 2 [l = D_a/4]; a non-empty J-row makes
 //    that a linear constraint, solved by as_eliminate on its last member.
 // Returns eps; p is the exponent of sqrt2, m the phase (mod 8, includes Q).
+// Packed layout, branch-free on the lanes: basis change of pivot i over Sp
+// (as as_basis_change), then z_i = c.  Rows b in Sp: J_b ^= Jc,
+// D_b += D_i + 4 J_bi (and G_b ^= G_i when WITHG); rows x in M: J_x ^= Sp;
+// c = 1 adds D_i to Q, 4 to D_x on M (and G_i to h when WITHG).  The caller
+// retires row i.
+template <bool WITHG> DEV void pk_eliminate(AState &s, int i, u64 Sp, unsigned c) {
+    const u64 Jr = rl64(s.J, i);
+    const unsigned di = (unsigned)(Jr >> 61);
+    const u64 jii = (u64)((di >> 1) & 1u);
+    const u64 Jc = (Jr & CSE_ROWMASK & ~bit(i)) | (jii << i);
+    const u64 M = Jc ^ (Sp & (0ull - jii));
+    const bool inS = lane_in(Sp), inSJ = lane_in(Sp & Jc), inM = lane_in(M);
+    const unsigned jlo = (unsigned)s.J, jhi = (unsigned)(s.J >> 32);
+    const u64 X = (inS ? Jc : 0ull) ^ (inM ? Sp : 0ull);
+    unsigned nhi = (jhi ^ (unsigned)(X >> 32)) + (inS ? (di << 29) : 0u);
+    nhi ^= (inSJ ? 0x80000000u : 0u) ^ ((inM && c) ? 0x80000000u : 0u);
+    s.J = ((u64)nhi << 32) | (jlo ^ (unsigned)X);
+    s.Q = (s.Q + (int)(di & (0u - c))) & 7;
+    if (WITHG) {
+        const u64 Gi = rl64(s.G, i);
+        s.G = inS ? (s.G ^ Gi) : s.G;
+        s.h ^= Gi & (0ull - (u64)c);
+    }
+}
+
+// as_restrict on the packed layout for xi = e_q: {x : x_q = alpha}.  The
+// rows in B (variables fixed to 1 and retired beforehand, never modified
+// since) still contribute to x_q: their part of h is the parity of column q
+// over B, taken from the same ballot as the pivot set.
+DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
+    const u64 col = ballot((s.G >> q) & 1ull);
+    const u64 S = col & s.act;
+    const unsigned beta = (alpha ^ (unsigned)(s.h >> q) ^ (unsigned)__popcll(col & B)) & 1u;
+    if (!S) return beta == 0;
+    const int i = __builtin_ctzll(S);
+    pk_eliminate<true>(s, i, S & ~bit(i), beta);
+    s.act &= ~bit(i);
+    return true;
+}
+
+//
+// Packed layout (D_x in bits 61..63 of J_x), the elimination loop scheduled
+// by hand: the compiler's lowering of this loop spends most of its scalar
+// instructions on control-flow bookkeeping, and scalar issue bounds the
+// sweep.  The loop takes, per step, the lowest active variable a:
+//  D_a in {2,6} (bit 62 of J_a): factor sqrt2 w^{s}, s = +1 for 2, -1 for 6;
+//    rows b in eta = J_a & act get J_b ^= eta and D_b -= D_a;
+//  D_a in {0,4}, eta != 0: factor 2 and the constraint eta . z = D_a/4,
+//    solved for i = min(eta) with Sp = eta \ {i} (pk_eliminate, c = D_a/4);
+//  D_a in {0,4}, eta = 0: factor 2 (D_a = 0) or the sum vanishes (D_a = 4).
+// Column i and row i are dead after a step, so Jc keeps the stored diagonal
+// bit of row i and M may contain i.  With nA, nB, nC the numbers of steps of
+// each kind over n0 variables (n0 = nA + 2 nB + nC) the result is
+// 2^{(n0 + nC)/2} w^m, m = Q + 2 n0 - 4 nB - 2 nC + sum_B c D_i - sum_A D_a/2.
+// Register use: s86 pivot index, then D << 29 (s87 = 0 makes s[86:87] its
+// 64-bit form), s[88:89] act, s90 W = bits 29..31: m * 2^29 without the
+// last sum, bit 28: the sum vanished, bits 0..27: nC; s[92:93] J_a,
+// s[94:95] eta / Sp, s[96:97] Jc, s[98:99] Sp & Jc, then M.  VALU operands
+// are read only from SGPRs written by SALU instructions.
+DEV int as_expsum_packed(AState &s, int &p, int &m) {
+    u64 act = uniform64(s.act);
+    s.act = 0;
+    if (!act) {
+        p = 0;
+        m = s.Q & 7;
+        return 1;
+    }
+    const int n0 = __popcll(act);
+    unsigned jlo = (unsigned)s.J, jhi = (unsigned)(s.J >> 32);
+    unsigned W = (unsigned)__builtin_amdgcn_readfirstlane((int)((unsigned)(s.Q + 2 * n0) << 29));
+    u64 acc = 0;  // sum over the {2,6} steps of D_a << 29, in every lane
+    unsigned vt0, vt1;
+    const unsigned vsign = 0x80000000u;
+    asm volatile(
+        "s_mov_b32 s87, 0\n"
+        "L_top%=:\n\t"
+        "s_ff1_i32_b64 s86, s[88:89]\n\t"
+        "s_bitset0_b64 s[88:89], s86\n\t"
+        "v_readlane_b32 s92, %[jlo], s86\n\t"
+        "v_readlane_b32 s93, %[jhi], s86\n\t"
+        "s_and_b64 s[94:95], s[92:93], s[88:89]\n\t"
+        "s_bitcmp1_b32 s93, 30\n\t"
+        "s_cbranch_scc0 L_nonA%=\n\t"
+        // D_a in {2,6}
+        "s_and_b32 s86, s93, 0xe0000000\n\t"
+        "v_xor_b32 %[vt0], s94, %[jlo]\n\t"
+        "v_xor_b32 %[vt1], s95, %[jhi]\n\t"
+        "v_subrev_u32 %[vt1], s86, %[vt1]\n\t"
+        "v_cndmask_b32_e64 %[jlo], %[jlo], %[vt0], s[94:95]\n\t"
+        "v_cndmask_b32_e64 %[jhi], %[jhi], %[vt1], s[94:95]\n\t"
+        "v_lshl_add_u64 %[acc], s[86:87], 0, %[acc]\n\t"
+        "s_cmp_lg_u64 s[88:89], 0\n\t"
+        "s_cbranch_scc1 L_top%=\n\t"
+        "s_branch L_end%=\n"
+        "L_nonA%=:\n\t"
+        "s_cmp_eq_u64 s[94:95], 0\n\t"
+        "s_cbranch_scc1 L_C%=\n\t"
+        // D_a in {0,4} with a constraint: pivot i = min(eta), Sp = eta \ i
+        "s_ff1_i32_b64 s86, s[94:95]\n\t"
+        "s_bitset0_b64 s[88:89], s86\n\t"
+        "s_bitset0_b64 s[94:95], s86\n\t"
+        "v_readlane_b32 s96, %[jlo], s86\n\t"
+        "v_readlane_b32 s97, %[jhi], s86\n\t"
+        "s_and_b32 s86, s97, 0xe0000000\n\t"
+        "s_and_b32 s97, s97, 0x1fffffff\n\t"
+        "s_and_b64 s[98:99], s[96:97], s[94:95]\n\t"
+        // rows in Sp: J ^= Jc, D += D_i; rows in Sp & Jc: D += 4
+        "v_mov_b32 %[vt0], s96\n\t"
+        "v_mov_b32 %[vt1], s97\n\t"
+        "v_cndmask_b32_e64 %[vt0], 0, %[vt0], s[94:95]\n\t"
+        "v_cndmask_b32_e64 %[vt1], 0, %[vt1], s[94:95]\n\t"
+        "v_xor_b32 %[jlo], %[jlo], %[vt0]\n\t"
+        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
+        "v_mov_b32 %[vt0], s86\n\t"
+        "v_cndmask_b32_e64 %[vt0], 0, %[vt0], s[94:95]\n\t"
+        "v_add_u32 %[jhi], %[jhi], %[vt0]\n\t"
+        "v_cndmask_b32_e64 %[vt1], 0, %[vsign], s[98:99]\n\t"
+        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
+        // M = Jc, ^ Sp when D_i in {2,6}; rows in M: J ^= Sp
+        "s_mov_b64 s[98:99], s[96:97]\n\t"
+        "s_bitcmp1_b32 s86, 30\n\t"
+        "s_cbranch_scc0 L_nojii%=\n\t"
+        "s_xor_b64 s[98:99], s[98:99], s[94:95]\n"
+        "L_nojii%=:\n\t"
+        "v_mov_b32 %[vt0], s94\n\t"
+        "v_mov_b32 %[vt1], s95\n\t"
+        "v_cndmask_b32_e64 %[vt0], 0, %[vt0], s[98:99]\n\t"
+        "v_cndmask_b32_e64 %[vt1], 0, %[vt1], s[98:99]\n\t"
+        "v_xor_b32 %[jlo], %[jlo], %[vt0]\n\t"
+        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
+        "s_add_u32 s90, s90, 0x80000000\n\t"
+        // z_i = c = D_a / 4: Q += D_i, rows in M: D += 4
+        "s_bitcmp1_b32 s93, 31\n\t"
+        "s_cbranch_scc0 L_noc%=\n\t"
+        "v_cndmask_b32_e64 %[vt1], 0, %[vsign], s[98:99]\n\t"
+        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
+        "s_add_u32 s90, s90, s86\n"
+        "L_noc%=:\n\t"
+        "s_cmp_lg_u64 s[88:89], 0\n\t"
+        "s_cbranch_scc1 L_top%=\n\t"
+        "s_branch L_end%=\n"
+        // D_a in {0,4} without a constraint: nC += 1, m -= 2
+        "L_C%=:\n\t"
+        "s_add_u32 s90, s90, 0xc0000001\n\t"
+        "s_bitcmp1_b32 s93, 31\n\t"
+        "s_cbranch_scc1 L_zero%=\n\t"
+        "s_cmp_lg_u64 s[88:89], 0\n\t"
+        "s_cbranch_scc1 L_top%=\n\t"
+        "s_branch L_end%=\n"
+        "L_zero%=:\n\t"
+        "s_or_b32 s90, s90, 0x10000000\n"
+        "L_end%=:\n"
+        : "+{s[88:89]}"(act), "+{s90}"(W), [jlo] "+v"(jlo), [jhi] "+v"(jhi), [acc] "+v"(acc),
+          [vt0] "=&v"(vt0), [vt1] "=&v"(vt1)
+        : [vsign] "v"(vsign)
+        : "s86", "s87", "s92", "s93", "s94", "s95", "s96", "s97", "s98", "s99", "scc");
+    if (W & 0x10000000u) {
+        p = 0;
+        m = 0;
+        return 0;
+    }
+    const unsigned half = (unsigned)(uniform64(acc) >> 30);
+    p = n0 + (int)(W & 0x0fffffffu);
+    m = (int)(((W >> 29) - half) & 7u);
+    return 1;
+}
+
 template <bool PK> DEV int as_expsum(AState &s, int &p, int &m) {
+    if (PK) return as_expsum_packed(s, p, m);
     int pp = 0, macc = 0, zero = 0;
     u64 act = s.act;
     while (act) {

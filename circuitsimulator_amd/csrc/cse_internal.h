@@ -81,16 +81,17 @@ struct RecView {
     u64 h, act, D1, D2;
     DEV void load(const uint64_t *rec) {
         const int x = lane_id();
-        const uint64_t w0 = rec[0];
+        const uint64_t w0 = uniform64(rec[0]);
         k = (int)(w0 & 0xff);
         n = (int)((w0 >> 8) & 0xff);
         Q = (int)((w0 >> 16) & 7);
         sexp = (int)((w0 >> 24) & 0xffff);
         vanished = (int)((w0 >> 40) & 1);
-        h = rec[1];
-        act = rec[2];
-        D1 = rec[3];
-        D2 = rec[4];
+        // the header is wave-uniform: read it into scalar registers
+        h = uniform64(rec[1]);
+        act = uniform64(rec[2]);
+        D1 = uniform64(rec[3]);
+        D2 = uniform64(rec[4]);
         if (vanished) return;
         G = rec[5 + x];
         J = rec[69 + x];

@@ -362,12 +362,14 @@ DEV int ip_phiL(const RecView &rv, u64 lowx, u64 xj, int n, int &p, int &m) {
     u64 Zn = Z & ~rv.act;
     if (Zn) {
         s.G = rv.G;
-        s.h = rv.h ^ wave_xor(lane_in(B) ? rv.G : 0ull);
-        do {
+        s.h = PK ? rv.h : rv.h ^ wave_xor(lane_in(B) ? rv.G : 0ull);
+        bool ok = true;
+        while (Zn && ok) {
             const int q = __ffsll((long long)Zn) - 1;
             Zn &= Zn - 1;
-            if (!as_restrict<PK>(s, bit(q), 0)) { p = 0; m = 0; return 0; }
-        } while (Zn);
+            ok = PK ? pk_restrict_coord(s, q, 0, B) : as_restrict<PK>(s, bit(q), 0);
+        }
+        if (!ok) { p = 0; m = 0; return 0; }
     }
     int pp, mm;
     const int eps = as_expsum<PK>(s, pp, mm);
@@ -570,24 +572,16 @@ DEV void sweep_body(const SweepArgs &A) {
         acc.clear();
         int parked = 0, cnt = 0;
         long long tslot = tbase;
-        // x_j of |L> incrementally: going from j-1 to j flips bits 0..c of j,
-        // c = ctz(j), i.e. rows k-1 .. k-1-c of L; lane c holds their XOR
-        u64 xj = D.exact ? 0ull : lrow_combo(D.L, D.k, (u64)j0);
-        u64 flip = 0;
-        if (!D.exact && x < D.k) {
-            for (int b = 0; b <= x; b++) flip ^= D.L[D.k - 1 - b];
-        }
-        for (long long j = j0; j < j1; j++) {
-            int p = 0, m = 0, eps;
-            if (j > j0) xj ^= rl64(flip, __ffsll(j) - 1);
-            if (D.exact) eps = ip_phiH<PK>(rv, (u64)j, D.n, p, m);
-            else eps = ip_phiL<PK>(rv, lowx, xj, D.n, p, m);
+        // instance fields used per term, held in registers
+        const int n = D.n, mode = A.mode;
+        // parks one term in lane cnt; every 64 terms the lanes fold them
+        auto emit = [&](int eps, int p, int m, bool offdiag) {
             int packed = 0;
             if (eps) {
                 if (p > 0 || p < -64) {
                     if (x == 0) atomicOr(A.err, 2);
                 } else {
-                    packed = 1 + ((-p) * 8 + (m & 7)) * 2 + ((A.mode == 1 && j != r) ? 1 : 0);
+                    packed = 1 + ((-p) * 8 + (m & 7)) * 2 + (offdiag ? 1 : 0);
                 }
             }
             if (lane_in(bit(cnt))) parked = packed;
@@ -596,9 +590,32 @@ DEV void sweep_body(const SweepArgs &A) {
                     const int v = parked - 1;
                     A.terms[tslot + x] = parked ? ((v >> 1) + 1) : 0;
                 }
-                fold_terms(parked, 64, A.mode, rv.sexp, acc, hist);
+                fold_terms(parked, 64, mode, rv.sexp, acc, hist);
                 tslot += 64;
                 cnt = 0;
+            }
+        };
+        if (D.exact) {
+            for (long long j = j0; j < j1; j++) {
+                int p = 0, m = 0;
+                const int eps = ip_phiH<PK>(rv, (u64)j, n, p, m);
+                emit(eps, p, m, mode == 1 && j != r);
+            }
+        } else {
+            // x_j of |L> incrementally: going from j-1 to j flips bits 0..c
+            // of j, c = ctz(j), i.e. rows k-1 .. k-1-c of L; lane c holds
+            // their XOR
+            const int k = D.k;
+            u64 xj = lrow_combo(D.L, k, (u64)j0);
+            u64 flip = 0;
+            if (x < k) {
+                for (int b = 0; b <= x; b++) flip ^= D.L[k - 1 - b];
+            }
+            for (long long j = j0; j < j1; j++) {
+                int p = 0, m = 0;
+                if (j > j0) xj ^= rl64(flip, __ffsll(j) - 1);
+                const int eps = ip_phiL<PK>(rv, lowx, xj, n, p, m);
+                emit(eps, p, m, mode == 1 && j != r);
             }
         }
         if (cnt) {
@@ -606,7 +623,7 @@ DEV void sweep_body(const SweepArgs &A) {
                 const int v = parked - 1;
                 A.terms[tslot + x] = parked ? ((v >> 1) + 1) : 0;
             }
-            fold_terms(parked, cnt, A.mode, rv.sexp, acc, hist);
+            fold_terms(parked, cnt, mode, rv.sexp, acc, hist);
         }
         // flush the item (the LDS histogram atomics of this wave come first)
         __threadfence_block();
