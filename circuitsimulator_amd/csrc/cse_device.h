@@ -437,14 +437,14 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
 //    solved for i = min(eta) with Sp = eta \ {i} (pk_eliminate, c = D_a/4);
 //  D_a in {0,4}, eta = 0: factor 2 (D_a = 0) or the sum vanishes (D_a = 4).
 // Column i and row i are dead after a step, so Jc keeps the stored diagonal
-// bit of row i and M may contain i.  With nA, nB, nC the numbers of steps of
+// bit of row i, M may contain i, and eta (which contains i) stands for Sp.  With nA, nB, nC the numbers of steps of
 // each kind over n0 variables (n0 = nA + 2 nB + nC) the result is
 // 2^{(n0 + nC)/2} w^m, m = Q + 2 n0 - 4 nB - 2 nC + sum_B c D_i - sum_A D_a/2.
 // Register use: s86 pivot index, then D << 29 (s87 = 0 makes s[86:87] its
 // 64-bit form), s[88:89] act, s90 W = bits 29..31: m * 2^29 without the
-// last sum, bit 28: the sum vanished, bits 0..27: nC; s[92:93] J_a,
-// s[94:95] eta / Sp, s[96:97] Jc, s[98:99] Sp & Jc, then M.  VALU operands
-// are read only from SGPRs written by SALU instructions.
+// last sum (acc), bit 28: the sum vanished, bits 0..27: nC; s[92:93] J_a,
+// s[94:95] eta, s[96:97] Jc.  A VALU instruction reads an SGPR that
+// v_readlane wrote only two or more instructions later.
 DEV int as_expsum_packed(AState &s, int &p, int &m) {
     u64 act = uniform64(s.act);
     s.act = 0;
@@ -456,9 +456,10 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
     const int n0 = __popcll(act);
     unsigned jlo = (unsigned)s.J, jhi = (unsigned)(s.J >> 32);
     unsigned W = (unsigned)__builtin_amdgcn_readfirstlane((int)((unsigned)(s.Q + 2 * n0) << 29));
-    u64 acc = 0;  // sum over the {2,6} steps of D_a << 29, in every lane
-    unsigned vt0, vt1;
-    const unsigned vsign = 0x80000000u;
+    // in every lane: sum over the {2,6} steps of D_a << 29, plus 2^32 per
+    // {0,4} step with a constraint (the -4 nB of m)
+    u64 acc = 0;
+    unsigned vt0, vt1, vt2, vt3;
     asm volatile(
         "s_mov_b32 s87, 0\n"
         "L_top%=:\n\t"
@@ -466,10 +467,10 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
         "s_bitset0_b64 s[88:89], s86\n\t"
         "v_readlane_b32 s92, %[jlo], s86\n\t"
         "v_readlane_b32 s93, %[jhi], s86\n\t"
-        "s_and_b64 s[94:95], s[92:93], s[88:89]\n\t"
         "s_bitcmp1_b32 s93, 30\n\t"
         "s_cbranch_scc0 L_nonA%=\n\t"
         // D_a in {2,6}
+        "s_and_b64 s[94:95], s[92:93], s[88:89]\n\t"
         "s_and_b32 s86, s93, 0xe0000000\n\t"
         "v_xor_b32 %[vt0], s94, %[jlo]\n\t"
         "v_xor_b32 %[vt1], s95, %[jhi]\n\t"
@@ -481,49 +482,46 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
         "s_cbranch_scc1 L_top%=\n\t"
         "s_branch L_end%=\n"
         "L_nonA%=:\n\t"
-        "s_cmp_eq_u64 s[94:95], 0\n\t"
-        "s_cbranch_scc1 L_C%=\n\t"
-        // D_a in {0,4} with a constraint: pivot i = min(eta), Sp = eta \ i
+        "s_and_b64 s[94:95], s[92:93], s[88:89]\n\t"
+        "s_cbranch_scc0 L_C%=\n\t"
+        // D_a in {0,4} with a constraint: pivot i = min(eta).  Row i and
+        // column i are dead from here on, so eta itself serves as Sp.
         "s_ff1_i32_b64 s86, s[94:95]\n\t"
         "s_bitset0_b64 s[88:89], s86\n\t"
-        "s_bitset0_b64 s[94:95], s86\n\t"
         "v_readlane_b32 s96, %[jlo], s86\n\t"
         "v_readlane_b32 s97, %[jhi], s86\n\t"
         "s_and_b32 s86, s97, 0xe0000000\n\t"
         "s_and_b32 s97, s97, 0x1fffffff\n\t"
-        "s_and_b64 s[98:99], s[96:97], s[94:95]\n\t"
-        // rows in Sp: J ^= Jc, D += D_i; rows in Sp & Jc: D += 4
-        "v_mov_b32 %[vt0], s96\n\t"
-        "v_mov_b32 %[vt1], s97\n\t"
-        "v_cndmask_b32_e64 %[vt0], 0, %[vt0], s[94:95]\n\t"
-        "v_cndmask_b32_e64 %[vt1], 0, %[vt1], s[94:95]\n\t"
-        "v_xor_b32 %[jlo], %[jlo], %[vt0]\n\t"
+        "v_cndmask_b32_e64 %[vt0], 0, -1, s[94:95]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[vt3], 0, -1, s[96:97]\n\t"
+        // rows in Sp: J ^= Jc, D += D_i, and D += 4 where J_xi = 1
+        "v_and_b32 %[vt1], s96, %[vt0]\n\t"
+        "v_xor_b32 %[jlo], %[jlo], %[vt1]\n\t"
+        "v_and_b32 %[vt1], s97, %[vt0]\n\t"
+        "v_and_b32 %[vt2], s86, %[vt0]\n\t"
+        "v_xad_u32 %[jhi], %[jhi], %[vt1], %[vt2]\n\t"
+        "v_and_b32 %[vt1], %[vt3], %[vt0]\n\t"
+        "v_and_b32 %[vt1], 0x80000000, %[vt1]\n\t"
         "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
-        "v_mov_b32 %[vt0], s86\n\t"
-        "v_cndmask_b32_e64 %[vt0], 0, %[vt0], s[94:95]\n\t"
-        "v_add_u32 %[jhi], %[jhi], %[vt0]\n\t"
-        "v_cndmask_b32_e64 %[vt1], 0, %[vsign], s[98:99]\n\t"
-        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
-        // M = Jc, ^ Sp when D_i in {2,6}; rows in M: J ^= Sp
-        "s_mov_b64 s[98:99], s[96:97]\n\t"
+        // lanes of M = Jc, ^ Sp when D_i in {2,6}
         "s_bitcmp1_b32 s86, 30\n\t"
         "s_cbranch_scc0 L_nojii%=\n\t"
-        "s_xor_b64 s[98:99], s[98:99], s[94:95]\n"
+        "v_xor_b32 %[vt3], %[vt3], %[vt0]\n"
         "L_nojii%=:\n\t"
-        "v_mov_b32 %[vt0], s94\n\t"
-        "v_mov_b32 %[vt1], s95\n\t"
-        "v_cndmask_b32_e64 %[vt0], 0, %[vt0], s[98:99]\n\t"
-        "v_cndmask_b32_e64 %[vt1], 0, %[vt1], s[98:99]\n\t"
-        "v_xor_b32 %[jlo], %[jlo], %[vt0]\n\t"
-        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
-        "s_add_u32 s90, s90, 0x80000000\n\t"
-        // z_i = c = D_a / 4: Q += D_i, rows in M: D += 4
+        // m -= 4 (acc += 2^32); z_i = c = D_a / 4: Q += D_i and D += 4 on M
+        // (folded into Sp.hi)
+        "v_lshl_add_u64 %[acc], %[four], 0, %[acc]\n\t"
         "s_bitcmp1_b32 s93, 31\n\t"
         "s_cbranch_scc0 L_noc%=\n\t"
-        "v_cndmask_b32_e64 %[vt1], 0, %[vsign], s[98:99]\n\t"
-        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
+        "s_or_b32 s95, s95, 0x80000000\n\t"
         "s_add_u32 s90, s90, s86\n"
         "L_noc%=:\n\t"
+        // rows in M: J ^= Sp
+        "v_and_b32 %[vt1], s94, %[vt3]\n\t"
+        "v_xor_b32 %[jlo], %[jlo], %[vt1]\n\t"
+        "v_and_b32 %[vt1], s95, %[vt3]\n\t"
+        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
         "s_cmp_lg_u64 s[88:89], 0\n\t"
         "s_cbranch_scc1 L_top%=\n\t"
         "s_branch L_end%=\n"
@@ -539,9 +537,9 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
         "s_or_b32 s90, s90, 0x10000000\n"
         "L_end%=:\n"
         : "+{s[88:89]}"(act), "+{s90}"(W), [jlo] "+v"(jlo), [jhi] "+v"(jhi), [acc] "+v"(acc),
-          [vt0] "=&v"(vt0), [vt1] "=&v"(vt1)
-        : [vsign] "v"(vsign)
-        : "s86", "s87", "s92", "s93", "s94", "s95", "s96", "s97", "s98", "s99", "scc");
+          [vt0] "=&v"(vt0), [vt1] "=&v"(vt1), [vt2] "=&v"(vt2), [vt3] "=&v"(vt3)
+        : [four] "v"(1ull << 32)
+        : "s86", "s87", "s92", "s93", "s94", "s95", "s96", "s97", "scc");
     if (W & 0x10000000u) {
         p = 0;
         m = 0;

@@ -337,29 +337,31 @@ k_records(int mode, int ninst, const long long *__restrict__ rec_off, const Inst
 // The sweep.
 namespace {
 
-// <phi_j | theta'> for phi_j = prepL(j) (a product of |0> and |+>): the
-// coordinates outside supp(x_j) are fixed to 0.  On pivot columns this fixes
-// the variable itself (x_a = h_a + z_a), all at once; the remaining zero
-// coordinates are restrictions.
+// Packed term of eps 2^{p/2} w^m: 0 for eps = 0, else 1 + 2 ((-p) 8 + m).
+DEV int pack_term(int eps, int np, int m) { return eps ? 1 + 2 * (np * 8 + (m & 7)) : 0; }
+
+// <phi_j | theta'> for phi_j = prepL(j) (a product of |0> and |+>), packed:
+// the coordinates outside supp(x_j) are fixed to 0.  On pivot columns this
+// fixes the variable itself (x_a = h_a + z_a), all at once; the remaining
+// zero coordinates (nonact = non-pivot columns of [0, n)) are restrictions.
 template <bool PK>
-DEV int ip_phiL(const RecView &rv, u64 lowx, u64 xj, int n, int &p, int &m) {
-    const u64 Z = ~xj & lowmask(n);
-    const u64 F = Z & rv.act;
+DEV int ip_phiL(const RecView &rv, u64 lowx, u64 xj, u64 nonact) {
+    const u64 F = rv.act & ~xj;
     const u64 B = F & rv.h;
     AState s;
     // Q += sum_{a in B} D_a + 4 #{edges of J inside B}; D_x += 4 |J_x & B|
-    int Q = rv.Q + 2 * __popcll(rv.D1 & B) + 4 * __popcll(rv.D2 & B);
+    // (the two 4 x parity terms share one popcount)
     const u64 JB = rv.J & B;
-    Q += 4 * (__popcll(ballot(lane_in(B) && par64(JB & lowx))) & 1);
-    s.Q = Q & 7;
+    const u64 E = ballot(par64(JB & lowx));
+    s.Q = rv.Q + 2 * __popcll(rv.D1 & B) + 4 * __popcll((rv.D2 ^ E) & B);
     if (PK) {
         s.J = rv.J ^ ((u64)par64(JB) << 63);
     } else {
         s.J = rv.J;
         s.d = rv.d ^ (par64(JB) << 2);
     }
-    s.act = rv.act & ~F;
-    u64 Zn = Z & ~rv.act;
+    s.act = rv.act & xj;
+    u64 Zn = nonact & ~xj;
     if (Zn) {
         s.G = rv.G;
         s.h = PK ? rv.h : rv.h ^ wave_xor(lane_in(B) ? rv.G : 0ull);
@@ -369,20 +371,19 @@ DEV int ip_phiL(const RecView &rv, u64 lowx, u64 xj, int n, int &p, int &m) {
             Zn &= Zn - 1;
             ok = PK ? pk_restrict_coord(s, q, 0, B) : as_restrict<PK>(s, bit(q), 0);
         }
-        if (!ok) { p = 0; m = 0; return 0; }
+        if (!ok) return 0;
     }
     int pp, mm;
     const int eps = as_expsum<PK>(s, pp, mm);
-    p = eps ? pp - rv.k - __popcll(xj) : 0;
-    m = mm;
-    return eps;
+    return pack_term(eps, rv.k + __popcll(xj) - pp, mm);
 }
 
-// <phi_j | theta'> for phi_j = prepH(j): pair constraints x_{2q} = x_{2q+1}
-// (bit 1), couplings 4 x_{2q} x_{2q+1} (bit 0), last qubit of odd t fixed to
-// 0 on bit 1 (stateprep.c:36-81, as a coordinate-form state).
+// <phi_j | theta'> for phi_j = prepH(j), packed: pair constraints
+// x_{2q} = x_{2q+1} (bit 1), couplings 4 x_{2q} x_{2q+1} (bit 0), last qubit
+// of odd t fixed to 0 on bit 1 (stateprep.c:36-81, as a coordinate-form
+// state).
 template <bool PK>
-DEV int ip_phiH(const RecView &rv, u64 j, int t, int &p, int &m) {
+DEV int ip_phiH(const RecView &rv, u64 j, int t) {
     AState s;
     s.G = rv.G;
     s.J = rv.J;
@@ -397,7 +398,7 @@ DEV int ip_phiH(const RecView &rv, u64 j, int t, int &p, int &m) {
         if (!b) continue;
         u64 xi = ((t & 1) && q == size - 1) ? bit(t - 1) : (bit(2 * q) | bit(2 * q + 1));
         k2--;
-        if (!as_restrict<PK>(s, xi, 0)) { p = 0; m = 0; return 0; }
+        if (!as_restrict<PK>(s, xi, 0)) return 0;
     }
     for (int q = 0; q < size; q++) {
         const int b = (int)((j >> (size - 1 - q)) & 1ull);
@@ -406,9 +407,7 @@ DEV int ip_phiH(const RecView &rv, u64 j, int t, int &p, int &m) {
     }
     int pp, mm;
     const int eps = as_expsum<PK>(s, pp, mm);
-    p = eps ? pp - rv.k - k2 : 0;
-    m = mm;
-    return eps;
+    return pack_term(eps, rv.k + k2 - pp, mm);
 }
 
 DEV u64 lrow_combo(const uint64_t *L, int k, u64 j) {
@@ -496,13 +495,17 @@ DEV double wave_sumf64(double v) {
 }
 
 // Fold the parked terms (lanes < cnt).  Packed term: 0 for eps = 0, else
-// 1 + ((-p) * 8 + m) * 2 + offdiag.
-DEV void fold_terms(int packed, int cnt, int mode, int sexp, LaneAcc &acc, unsigned int *hist) {
+// 1 + ((-p) * 8 + m) * 2; in mode 1 the term in lane diag is the diagonal
+// one and every other term counts as 2 Re.  Returns 1 in a lane whose term
+// is out of range (not folded).
+DEV int fold_terms(int packed, int cnt, int mode, int diag, int sexp, LaneAcc &acc, unsigned int *hist) {
     const int x = lane_id();
-    if (x >= cnt || packed == 0) return;
+    if (x >= cnt || packed == 0) return 0;
     const int v = packed - 1;
-    const int off = v & 1;
+    const int off = mode == 1 && x != diag;
     const int pm = v >> 1;
+    // a canonical inner product has -64 <= p <= 0; anything else is an error
+    if ((unsigned)pm >= (unsigned)LH_BINS) return 1;
     const int p = -(pm >> 3), m = pm & 7;
     float vr, vi;
     if (mode == 0) {
@@ -521,6 +524,7 @@ DEV void fold_terms(int packed, int cnt, int mode, int sexp, LaneAcc &acc, unsig
             atomicAdd(&hist[pm], 1u);
         }
     }
+    return 0;
 }
 
 }  // namespace
@@ -570,61 +574,53 @@ DEV void sweep_body(const SweepArgs &A) {
 
         LaneAcc acc;
         acc.clear();
-        int parked = 0, cnt = 0;
-        long long tslot = tbase;
         // instance fields used per term, held in registers
         const int n = D.n, mode = A.mode;
-        // parks one term in lane cnt; every 64 terms the lanes fold them
-        auto emit = [&](int eps, int p, int m, bool offdiag) {
-            int packed = 0;
-            if (eps) {
-                if (p > 0 || p < -64) {
-                    if (x == 0) atomicOr(A.err, 2);
-                } else {
-                    packed = 1 + ((-p) * 8 + (m & 7)) * 2 + (offdiag ? 1 : 0);
-                }
+        int bad = 0;  // per lane: a term outside p in [-64, 0] (never folded)
+        // a batch of up to 64 terms, one parked per lane, then folded by all
+        // lanes; in mode 1 every term but j = r counts twice (2 Re)
+        auto flush = [&](int parked, int nb, long long jb) {
+            if (A.terms && x < nb) {
+                const int v = parked - 1;
+                A.terms[tbase + (jb - j0) + x] = parked ? ((v >> 1) + 1) : 0;
             }
-            if (lane_in(bit(cnt))) parked = packed;
-            if (++cnt == 64) {
-                if (A.terms) {
-                    const int v = parked - 1;
-                    A.terms[tslot + x] = parked ? ((v >> 1) + 1) : 0;
-                }
-                fold_terms(parked, 64, mode, rv.sexp, acc, hist);
-                tslot += 64;
-                cnt = 0;
-            }
+            const int diag = (mode == 1 && r >= jb && r < jb + nb) ? (int)(r - jb) : -1;
+            bad |= fold_terms(parked, nb, mode, diag, rv.sexp, acc, hist);
         };
         if (D.exact) {
-            for (long long j = j0; j < j1; j++) {
-                int p = 0, m = 0;
-                const int eps = ip_phiH<PK>(rv, (u64)j, n, p, m);
-                emit(eps, p, m, mode == 1 && j != r);
+            for (long long jb = j0; jb < j1; jb += 64) {
+                const int nb = (int)(j1 - jb < 64 ? j1 - jb : 64);
+                int parked = 0;
+                for (int c = 0; c < nb; c++) {
+                    const int packed = ip_phiH<PK>(rv, (u64)(jb + c), n);
+                    if (lane_in(bit(c))) parked = packed;
+                }
+                flush(parked, nb, jb);
             }
         } else {
             // x_j of |L> incrementally: going from j-1 to j flips bits 0..c
             // of j, c = ctz(j), i.e. rows k-1 .. k-1-c of L; lane c holds
             // their XOR
             const int k = D.k;
+            const u64 nonact = lowmask(n) & ~rv.act;
             u64 xj = lrow_combo(D.L, k, (u64)j0);
             u64 flip = 0;
             if (x < k) {
                 for (int b = 0; b <= x; b++) flip ^= D.L[k - 1 - b];
             }
-            for (long long j = j0; j < j1; j++) {
-                int p = 0, m = 0;
-                if (j > j0) xj ^= rl64(flip, __ffsll(j) - 1);
-                const int eps = ip_phiL<PK>(rv, lowx, xj, n, p, m);
-                emit(eps, p, m, mode == 1 && j != r);
+            for (long long jb = j0; jb < j1; jb += 64) {
+                const int nb = (int)(j1 - jb < 64 ? j1 - jb : 64);
+                int parked = 0;
+                for (int c = 0; c < nb; c++) {
+                    const int packed = ip_phiL<PK>(rv, lowx, xj, nonact);
+                    if (lane_in(bit(c))) parked = packed;
+                    // 0 < j + 1 <= chi <= 2^30
+                    xj ^= rl64(flip, __builtin_ctz((unsigned)(jb + c + 1)));
+                }
+                flush(parked, nb, jb);
             }
         }
-        if (cnt) {
-            if (A.terms && x < cnt) {
-                const int v = parked - 1;
-                A.terms[tslot + x] = parked ? ((v >> 1) + 1) : 0;
-            }
-            fold_terms(parked, cnt, mode, rv.sexp, acc, hist);
-        }
+        if (ballot(bad) && x == 0) atomicOr(A.err, 2);
         // flush the item (the LDS histogram atomics of this wave come first)
         __threadfence_block();
         const double kr = wave_sumf64((double)acc.sr - (double)acc.cr);
