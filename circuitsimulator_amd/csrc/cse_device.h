@@ -440,10 +440,10 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
 // bit of row i, M may contain i, and eta (which contains i) stands for Sp.  With nA, nB, nC the numbers of steps of
 // each kind over n0 variables (n0 = nA + 2 nB + nC) the result is
 // 2^{(n0 + nC)/2} w^m, m = Q + 2 n0 - 4 nB - 2 nC + sum_B c D_i - sum_A D_a/2.
-// Register use: s86 pivot index, then D << 29 (s87 = 0 makes s[86:87] its
-// 64-bit form), s[88:89] act, s90 W = bits 29..31: m * 2^29 without the
-// last sum (acc), bit 28: the sum vanished, bits 0..27: nC; s[92:93] J_a,
-// s[94:95] eta, s[96:97] Jc.  A VALU instruction reads an SGPR that
+// Register use: s56 pivot index, then D << 29 (s57 = 0 makes s[56:57] its
+// 64-bit form), s[58:59] act, s60 W = bits 29..31: m * 2^29 without the
+// last sum (acc), bit 28: the sum vanished, bits 0..27: nC; s[62:63] J_a,
+// s[64:65] eta, s[66:67] Jc.  A VALU instruction reads an SGPR that
 // v_readlane wrote only two or more instructions later.
 DEV int as_expsum_packed(AState &s, int &p, int &m) {
     u64 act = uniform64(s.act);
@@ -461,85 +461,81 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
     u64 acc = 0;
     unsigned vt0, vt1, vt2, vt3;
     asm volatile(
-        "s_mov_b32 s87, 0\n"
+        "s_mov_b32 s57, 0\n"
         "L_top%=:\n\t"
-        "s_ff1_i32_b64 s86, s[88:89]\n\t"
-        "s_bitset0_b64 s[88:89], s86\n\t"
-        "v_readlane_b32 s92, %[jlo], s86\n\t"
-        "v_readlane_b32 s93, %[jhi], s86\n\t"
-        "s_bitcmp1_b32 s93, 30\n\t"
+        "s_ff1_i32_b64 s56, s[58:59]\n\t"
+        "s_bitset0_b64 s[58:59], s56\n\t"
+        "v_readlane_b32 s62, %[jlo], s56\n\t"
+        "v_readlane_b32 s63, %[jhi], s56\n\t"
+        "s_bitcmp1_b32 s63, 30\n\t"
         "s_cbranch_scc0 L_nonA%=\n\t"
         // D_a in {2,6}
-        "s_and_b64 s[94:95], s[92:93], s[88:89]\n\t"
-        "s_and_b32 s86, s93, 0xe0000000\n\t"
-        "v_xor_b32 %[vt0], s94, %[jlo]\n\t"
-        "v_xor_b32 %[vt1], s95, %[jhi]\n\t"
-        "v_subrev_u32 %[vt1], s86, %[vt1]\n\t"
-        "v_cndmask_b32_e64 %[jlo], %[jlo], %[vt0], s[94:95]\n\t"
-        "v_cndmask_b32_e64 %[jhi], %[jhi], %[vt1], s[94:95]\n\t"
-        "v_lshl_add_u64 %[acc], s[86:87], 0, %[acc]\n\t"
-        "s_cmp_lg_u64 s[88:89], 0\n\t"
+        "s_and_b64 s[64:65], s[62:63], s[58:59]\n\t"
+        "s_and_b32 s56, s63, 0xe0000000\n\t"
+        "v_xor_b32 %[vt0], s64, %[jlo]\n\t"
+        "v_xor_b32 %[vt1], s65, %[jhi]\n\t"
+        "v_subrev_u32 %[vt1], s56, %[vt1]\n\t"
+        "v_cndmask_b32_e64 %[jlo], %[jlo], %[vt0], s[64:65]\n\t"
+        "v_cndmask_b32_e64 %[jhi], %[jhi], %[vt1], s[64:65]\n\t"
+        "v_lshl_add_u64 %[acc], s[56:57], 0, %[acc]\n\t"
+        "s_cmp_lg_u64 s[58:59], 0\n\t"
         "s_cbranch_scc1 L_top%=\n\t"
         "s_branch L_end%=\n"
         "L_nonA%=:\n\t"
-        "s_and_b64 s[94:95], s[92:93], s[88:89]\n\t"
+        "s_and_b64 s[64:65], s[62:63], s[58:59]\n\t"
         "s_cbranch_scc0 L_C%=\n\t"
         // D_a in {0,4} with a constraint: pivot i = min(eta).  Row i and
         // column i are dead from here on, so eta itself serves as Sp.
-        "s_ff1_i32_b64 s86, s[94:95]\n\t"
-        "s_bitset0_b64 s[88:89], s86\n\t"
-        "v_readlane_b32 s96, %[jlo], s86\n\t"
-        "v_readlane_b32 s97, %[jhi], s86\n\t"
-        "s_and_b32 s86, s97, 0xe0000000\n\t"
-        "s_and_b32 s97, s97, 0x1fffffff\n\t"
-        "v_cndmask_b32_e64 %[vt0], 0, -1, s[94:95]\n\t"
+        "s_ff1_i32_b64 s56, s[64:65]\n\t"
+        "s_bitset0_b64 s[58:59], s56\n\t"
+        "v_readlane_b32 s66, %[jlo], s56\n\t"
+        "v_readlane_b32 s67, %[jhi], s56\n\t"
+        "s_and_b32 s56, s67, 0xe0000000\n\t"
+        "s_and_b32 s67, s67, 0x1fffffff\n\t"
+        "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t"
         "s_nop 1\n\t"
-        "v_cndmask_b32_e64 %[vt3], 0, -1, s[96:97]\n\t"
-        // rows in Sp: J ^= Jc, D += D_i, and D += 4 where J_xi = 1
-        "v_and_b32 %[vt1], s96, %[vt0]\n\t"
-        "v_xor_b32 %[jlo], %[jlo], %[vt1]\n\t"
-        "v_and_b32 %[vt1], s97, %[vt0]\n\t"
-        "v_and_b32 %[vt2], s86, %[vt0]\n\t"
-        "v_xad_u32 %[jhi], %[jhi], %[vt1], %[vt2]\n\t"
-        "v_and_b32 %[vt1], %[vt3], %[vt0]\n\t"
-        "v_and_b32 %[vt1], 0x80000000, %[vt1]\n\t"
-        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
+        "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"
+        // rows in Sp: J ^= Jc, D += D_i, and D += 4 where J_xi = 1.
+        // bitop3 0x78 is S0 ^ (S1 & S2), 0x60 is S0 & (S1 ^ S2).
+        "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"
+        "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"
+        "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t"
+        "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t"
+        "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t"
         // lanes of M = Jc, ^ Sp when D_i in {2,6}
-        "s_bitcmp1_b32 s86, 30\n\t"
+        "s_bitcmp1_b32 s56, 30\n\t"
         "s_cbranch_scc0 L_nojii%=\n\t"
         "v_xor_b32 %[vt3], %[vt3], %[vt0]\n"
         "L_nojii%=:\n\t"
         // m -= 4 (acc += 2^32); z_i = c = D_a / 4: Q += D_i and D += 4 on M
         // (folded into Sp.hi)
         "v_lshl_add_u64 %[acc], %[four], 0, %[acc]\n\t"
-        "s_bitcmp1_b32 s93, 31\n\t"
+        "s_bitcmp1_b32 s63, 31\n\t"
         "s_cbranch_scc0 L_noc%=\n\t"
-        "s_or_b32 s95, s95, 0x80000000\n\t"
-        "s_add_u32 s90, s90, s86\n"
+        "s_or_b32 s65, s65, 0x80000000\n\t"
+        "s_add_u32 s60, s60, s56\n"
         "L_noc%=:\n\t"
         // rows in M: J ^= Sp
-        "v_and_b32 %[vt1], s94, %[vt3]\n\t"
-        "v_xor_b32 %[jlo], %[jlo], %[vt1]\n\t"
-        "v_and_b32 %[vt1], s95, %[vt3]\n\t"
-        "v_xor_b32 %[jhi], %[jhi], %[vt1]\n\t"
-        "s_cmp_lg_u64 s[88:89], 0\n\t"
+        "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"
+        "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t"
+        "s_cmp_lg_u64 s[58:59], 0\n\t"
         "s_cbranch_scc1 L_top%=\n\t"
         "s_branch L_end%=\n"
         // D_a in {0,4} without a constraint: nC += 1, m -= 2
         "L_C%=:\n\t"
-        "s_add_u32 s90, s90, 0xc0000001\n\t"
-        "s_bitcmp1_b32 s93, 31\n\t"
+        "s_add_u32 s60, s60, 0xc0000001\n\t"
+        "s_bitcmp1_b32 s63, 31\n\t"
         "s_cbranch_scc1 L_zero%=\n\t"
-        "s_cmp_lg_u64 s[88:89], 0\n\t"
+        "s_cmp_lg_u64 s[58:59], 0\n\t"
         "s_cbranch_scc1 L_top%=\n\t"
         "s_branch L_end%=\n"
         "L_zero%=:\n\t"
-        "s_or_b32 s90, s90, 0x10000000\n"
+        "s_or_b32 s60, s60, 0x10000000\n"
         "L_end%=:\n"
-        : "+{s[88:89]}"(act), "+{s90}"(W), [jlo] "+v"(jlo), [jhi] "+v"(jhi), [acc] "+v"(acc),
+        : "+{s[58:59]}"(act), "+{s60}"(W), [jlo] "+v"(jlo), [jhi] "+v"(jhi), [acc] "+v"(acc),
           [vt0] "=&v"(vt0), [vt1] "=&v"(vt1), [vt2] "=&v"(vt2), [vt3] "=&v"(vt3)
         : [four] "v"(1ull << 32)
-        : "s86", "s87", "s92", "s93", "s94", "s95", "s96", "s97", "scc");
+        : "s56", "s57", "s62", "s63", "s64", "s65", "s66", "s67", "scc");
     if (W & 0x10000000u) {
         p = 0;
         m = 0;

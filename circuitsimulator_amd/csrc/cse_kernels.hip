@@ -659,7 +659,7 @@ DEV void sweep_body(const SweepArgs &A) {
 extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_body<false>(A); }
 
 // the same sweep with D packed into the J rows (every instance has n <= 61)
-extern "C" __global__ void __launch_bounds__(256) k_sweep_packed(SweepArgs A) { sweep_body<true>(A); }
+extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed(SweepArgs A) { sweep_body<true>(A); }
 
 extern "C" __global__ void k_zero64(unsigned long long *p, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
