@@ -101,6 +101,71 @@ def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0):
             "sample": f"{n} inner products theta' x prepL at t={t} by the oracle, 1 thread, {dt:.1f} s"}
 
 
+def exact_pairs_config(args):
+    """C2, reading B (the reference-native loop, innerprod.c:148-261): exact
+    mode at t=24, r=12, k=11, chi=2048, so 2,098,176 pair terms, each one
+    inner product <phi_j|Pi phi_i>.  value = pair terms/s of the whole call."""
+    from circuitsimulator_amd import Engine
+    from circuitsimulator_amd.workloads import sampled_config
+    t, r, k = 24, 12, 11
+    P, P_tuple, L = sampled_config(t, r, k, seed=1002)
+    chi = 1 << k
+    nterms = chi * (chi + 1) // 2
+    with Engine(0) as E:
+        for _ in range(args.warmup):
+            E.exact_projector(P, L, k, 0)
+        t0 = time.perf_counter()
+        kernel_ms = 0.0
+        for _ in range(args.steps):
+            res, _ = E.exact_projector(P, L, k, 0)
+            kernel_ms += res.kernel_ms
+        dt = time.perf_counter() - t0
+    return {
+        "metric": "exact projector pair terms/s (one inner product each), t=24 r=12 k=11",
+        "value": nterms * args.steps / dt, "unit": "inner products/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1000.0 / args.steps, "higher_is_better": True,
+        "dtype": "int64", "data": "synthetic (seeded commuting Pauli group and L)",
+        "config": {"workload": f"C2 reading B: exact projector t={t} r={r} k={k}, {nterms} pair terms",
+                   "t": t, "r": r, "k": k, "chi": chi, "seed": 1002},
+        "kernel_ip_per_s": nterms * args.steps / (kernel_ms / 1000.0),
+        "result": {"value_int64_exact": res.value, "re": res.re, "im": res.im,
+                   "kahan_fp32": [res.kahan_re, res.kahan_im],
+                   "abs_err_kahan_fp32": abs(complex(res.kahan_re, res.kahan_im) - complex(res.re, res.im))},
+    }
+
+
+def batch_config(args):
+    """C3: 4096 uneven exact-mode instances (t ~ U{8..48}, r ~ U{1..t/2},
+    k ~ U{2..14}), every pair term of every instance in one persistent
+    launch.  value = pair terms/s of the whole call."""
+    from circuitsimulator_amd import Engine
+    from circuitsimulator_amd.workloads import uneven_instances
+    count = args.instances
+    inst, specs = uneven_instances(count, seed=1003)
+    nterms = sum((1 << k) * ((1 << k) + 1) // 2 for (_, _, k, *_rest) in specs)
+    with Engine(0) as E:
+        for _ in range(args.warmup):
+            E.batch_exact(inst)
+        t0 = time.perf_counter()
+        kernel_ms = 0.0
+        for _ in range(args.steps):
+            res, _, ms = E.batch_exact(inst)
+            kernel_ms += ms
+        dt = time.perf_counter() - t0
+    ims = [abs(r.im) for r in res]
+    return {
+        "metric": "batched uneven exact instances: pair terms/s (one inner product each)",
+        "value": nterms * args.steps / dt, "unit": "inner products/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1000.0 / args.steps, "higher_is_better": True,
+        "dtype": "int64", "data": "synthetic (seeded per-instance commuting Pauli groups and L)",
+        "config": {"workload": f"C3: {count} instances t~U{{8..48}} r~U{{1..t/2}} k~U{{2..14}}, "
+                               f"{nterms} pair terms", "instances": count, "seed": 1003},
+        "kernel_ip_per_s": nterms * args.steps / (kernel_ms / 1000.0),
+        "checks": {"max_abs_im": max(ims), "min_value": min(r.re for r in res),
+                   "terms_reported": int(sum(r.terms for r in res)), "terms_expected": nterms},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,12 +175,20 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--plain-pairs", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config", choices=["c4", "c2", "c3"], default="c4",
+                    help="c4: the headline norm estimation (default); c2, c3: the other GPU configurations")
+    ap.add_argument("--instances", type=int, default=4096, help="c3: number of instances")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this engine is measured on one GPU; multi-GPU runs are not part of this build "
               "(the per-GPU sharding design is described in the design notes).", file=sys.stderr)
         sys.exit(2)
+
+    if args.config != "c4":
+        out = exact_pairs_config(args) if args.config == "c2" else batch_config(args)
+        print(json.dumps(out), flush=True)
+        return
 
     import numpy as np
 

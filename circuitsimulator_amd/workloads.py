@@ -68,6 +68,8 @@ def gf2_rank(rows):
 
 
 def random_L(rng, t, k):
+    if not 0 <= k <= t:
+        raise ValueError(f"a {k} x {t} L cannot have rank {k}")
     while True:
         L = [_bits(rng, t) for _ in range(k)]
         if gf2_rank(L) == k:
@@ -84,13 +86,14 @@ def sampled_config(t=48, r=24, k=12, seed=1004):
 
 def uneven_instances(count, seed=1003, tmin=8, tmax=48, kmin=2, kmax=14):
     """C3: independent exact-mode instances, t ~ U{tmin..tmax},
-    r ~ U{1..floor(t/2)}, k ~ U{kmin..kmax}, each with its own group and L."""
+    r ~ U{1..floor(t/2)}, k ~ U{kmin..min(kmax, t)} (L has rank k, so k <= t,
+    as decompose clamps k to t), each with its own group and L."""
     rng = np.random.default_rng(seed)
     out, specs = [], []
     for _ in range(count):
         t = int(rng.integers(tmin, tmax + 1))
         r = int(rng.integers(1, t // 2 + 1))
-        k = int(rng.integers(kmin, kmax + 1))
+        k = int(rng.integers(kmin, min(kmax, t) + 1))
         ph, xs, zs = commuting_paulis(rng, t, r)
         L = random_L(rng, t, k)
         inst = Instance()
