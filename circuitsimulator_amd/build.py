@@ -45,10 +45,12 @@ def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     host_o = os.path.join(BUILD, "cse_host.o")
     eng_o = os.path.join(BUILD, "cse_engine.o")
+    norms_o = os.path.join(BUILD, "cse_norms.o")
     _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "cse_host.c"), "-o", host_o], not verbose)
+    _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "cse_norms.c"), "-o", norms_o], not verbose)
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "cse_engine.hip"), "-o", eng_o], not verbose)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, eng_o, host_o, "-lm"], not verbose)
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, eng_o, host_o, norms_o, "-lm"], not verbose)
     os.replace(tmp, LIB)
     btmp = BACKEND + ".tmp"
     _run(["gcc", *C_FLAGS, "-o", btmp, os.path.join(CSRC, "backend_main.c"),

@@ -86,78 +86,17 @@ static int mpi_rank(void)
 
 static cse_ctx *g_ctx = NULL;
 
-static cse_ctx *engine(void)
+static double projector_norm(const cse_projector *P, const uint64_t *L, int k, int exact, double norm,
+                             int sampled, int samples, int bins, uint64_t seed)
 {
-    if (!g_ctx) {
-        int dev = 0;
-        const char *s = getenv("CSE_DEVICE");
-        if (s && *s) dev = atoi(s);
-        int rc = cse_create(dev, &g_ctx);
-        if (rc != CSE_OK) die(4, "no usable gfx950 GPU for the engine", g_ctx ? cse_error(g_ctx) : NULL);
-    }
-    return g_ctx;
-}
-
-static double clifford_sum(const cse_projector *P)
-{
-    double sum = 1; /* include identity */
-    for (int g = 0; g < P->nstabs; g++) {
-        if (P->phase[g] == 0) sum += 1;
-        if (P->phase[g] == 2) sum -= 1;
-    }
-    return sum;
-}
-
-/* innerprod.c:148-199 */
-static double exact_projector(const cse_projector *P, const uint64_t *L, int k, int exact, double norm)
-{
-    if (P->nstabs == 0) return pow(norm, 2);
-    if (P->nqubits == 0) return clifford_sum(P) / (1 + (double)P->nstabs);
-    cse_exact_result r;
-    memset(&r, 0, sizeof(r));
-    int rc = cse_exact_projector(engine(), P, L, k, exact, &r);
-    if (rc != CSE_OK) die(5, "engine failure:", cse_error(engine()));
-    return r.value;
-}
-
-/* innerprod.c:45-84, with the sample range [first, first + samples) */
-static double sampled_projector(const cse_projector *P, const uint64_t *L, int k, int exact, double norm,
-                                int64_t samples, uint64_t seed, int64_t first)
-{
-    if (P->nstabs == 0) return pow(norm, 2);
-    if (P->nqubits == 0) return pow(norm, 2) * clifford_sum(P) / (1 + (double)P->nstabs);
-    if (samples <= 0) return 0.0 / 0.0;
-    double total = 0;
-    const int64_t step = 1 << 20;
-    for (int64_t s0 = 0; s0 < samples; s0 += step) {
-        int64_t cnt = samples - s0 < step ? samples - s0 : step;
-        cse_sampled_result r;
-        memset(&r, 0, sizeof(r));
-        int rc = cse_sampled_projector(engine(), P, L, k, exact, seed, first + s0, cnt, 0, &r);
-        if (rc != CSE_OK) die(5, "engine failure:", cse_error(engine()));
-        total += r.mean * (double)cnt;
-    }
-    return total / (double)samples;
-}
-
-static int cmp_double(const void *a, const void *b)
-{
-    double x = *(const double *)a, y = *(const double *)b;
-    return (x > y) - (x < y);
-}
-
-/* innerprod.c:23-41 */
-static double multi_sampled(const cse_projector *P, const uint64_t *L, int k, int exact, double norm,
-                            int samples, int bins, uint64_t seed)
-{
-    if (bins <= 1) return sampled_projector(P, L, k, exact, norm, samples, seed, 0);
-    double *vals = (double *)malloc(sizeof(double) * (size_t)bins);
-    for (int b = 0; b < bins; b++)
-        vals[b] = sampled_projector(P, L, k, exact, norm, samples, seed, (int64_t)b * samples);
-    qsort(vals, (size_t)bins, sizeof(double), cmp_double);
-    double out = (bins % 2) ? vals[(bins - 1) / 2] : (vals[bins / 2] + vals[bins / 2 - 1]) / 2;
-    free(vals);
-    return out;
+    int dev = 0;
+    const char *s = getenv("CSE_DEVICE");
+    if (s && *s) dev = atoi(s);
+    double v = 0;
+    const int rc = cse_projector_norm(&g_ctx, dev, P, L, k, exact, norm, sampled, samples, bins, seed, &v);
+    if (rc == CSE_ERR_NODEV) die(4, "no usable gfx950 GPU for the engine", g_ctx ? cse_error(g_ctx) : NULL);
+    if (rc != CSE_OK) die(5, "engine failure:", g_ctx ? cse_error(g_ctx) : NULL);
+    return v;
 }
 
 int main(int argc, char *argv[])
@@ -233,12 +172,9 @@ int main(int argc, char *argv[])
         else printf("Stabilizer rank of |L>: 2^%d\n", k);
     }
 
-    if (noapprox == 0 && forceSample == 0) {
-        double lim = exact ? pow(2, (t + 1) / 2) - 1 : pow(2, k) - 1;
-        if ((double)samples * (double)bins * 2 > lim) {
-            noapprox = 1;
-            if (verbose) printf("More samples than terms in exact calculation. Disabling sampling.\n");
-        }
+    if (!noapprox && !cse_sampling_kept(noapprox, forceSample, exact, t, k, samples, bins)) {
+        noapprox = 1;
+        if (verbose) printf("More samples than terms in exact calculation. Disabling sampling.\n");
     }
 
     if (exact == 0 && t > 0) {
@@ -253,14 +189,10 @@ int main(int argc, char *argv[])
         printf("]\n");
     }
 
-    double numerator, denominator;
-    if (noapprox == 0) {
-        numerator = multi_sampled(&G, L, k, exact, norm, samples, bins, seed);
-        denominator = multi_sampled(&H, L, k, exact, norm, samples, bins, seed ^ 0x9E3779B97F4A7C15ull);
-    } else {
-        numerator = exact_projector(&G, L, k, exact, norm);
-        denominator = exact_projector(&H, L, k, exact, norm);
-    }
+    const int sampled = noapprox == 0;
+    const double numerator = projector_norm(&G, L, k, exact, norm, sampled, samples, bins, seed);
+    const double denominator =
+        projector_norm(&H, L, k, exact, norm, sampled, samples, bins, seed ^ 0x9E3779B97F4A7C15ull);
 
     printf("|| Gprime |H^t> ||^2 ~= %.17e\n", numerator);
     printf("|| Hprime |H^t> ||^2 ~= %.17e\n", denominator);

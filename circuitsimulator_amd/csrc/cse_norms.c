@@ -1,0 +1,104 @@
+/*
+ * cse_norms.c -- the backend's per-projector computation, shared by the
+ * drop-in executable and the Python front end: ||P|H^t>||^2 exactly
+ * (innerprod.c:148-199) or as a median of means of sampled estimates
+ * (innerprod.c:23-84), with the empty-projector and t = 0 closed forms, and
+ * the sampling-disable rule of probability.c:162-174.  The engine context is
+ * created on the first call that needs the GPU.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "cse.h"
+
+static int ensure_ctx(cse_ctx **ctx, int device)
+{
+    if (*ctx) return CSE_OK;
+    int rc = cse_create(device, ctx);
+    return rc == CSE_OK ? CSE_OK : CSE_ERR_NODEV;
+}
+
+/* 1 + #{+I} - #{-I} over the generators (innerprod.c:157-167) */
+static double clifford_sum(const cse_projector *P)
+{
+    double sum = 1;
+    for (int g = 0; g < P->nstabs; g++) {
+        if (P->phase[g] == 0) sum += 1;
+        if (P->phase[g] == 2) sum -= 1;
+    }
+    return sum;
+}
+
+/* mean over samples [first, first + count), in launches of at most 2^20 */
+static int sampled_mean(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k, int exact,
+                        int64_t count, uint64_t seed, int64_t first, double *out)
+{
+    double total = 0;
+    const int64_t step = 1 << 20;
+    for (int64_t s0 = 0; s0 < count; s0 += step) {
+        const int64_t cnt = count - s0 < step ? count - s0 : step;
+        cse_sampled_result r;
+        memset(&r, 0, sizeof(r));
+        const int rc = cse_sampled_projector(ctx, P, L, k, exact, seed, first + s0, cnt, 0, &r);
+        if (rc != CSE_OK) return rc;
+        total += r.mean * (double)cnt;
+    }
+    *out = total / (double)count;
+    return CSE_OK;
+}
+
+static int cmp_double(const void *a, const void *b)
+{
+    const double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+int cse_sampling_kept(int noapprox, int forceSample, int exact, int t, int k, int samples, int bins)
+{
+    if (noapprox) return 0;
+    if (forceSample) return 1;
+    const double lim = exact ? pow(2, (t + 1) / 2) - 1 : pow(2, k) - 1;
+    return (double)samples * (double)bins * 2 > lim ? 0 : 1;
+}
+
+int cse_projector_norm(cse_ctx **ctx, int device, const cse_projector *P, const uint64_t *L, int k, int exact,
+                       double norm, int sampled, int samples, int bins, uint64_t seed, double *value)
+{
+    if (!ctx || !P || !value) return CSE_ERR_ARG;
+    if (P->nstabs == 0) {
+        *value = norm * norm;
+        return CSE_OK;
+    }
+    if (P->nqubits == 0) {
+        /* the sampled form keeps the norm factor, the exact one does not */
+        const double f = clifford_sum(P) / (1 + (double)P->nstabs);
+        *value = sampled ? norm * norm * f : f;
+        return CSE_OK;
+    }
+    if (sampled && (samples <= 0 || bins <= 0)) {
+        /* the reference divides 0 by 0 here (innerprod.c:83) */
+        *value = NAN;
+        return CSE_OK;
+    }
+    int rc = ensure_ctx(ctx, device);
+    if (rc != CSE_OK) return rc;
+    if (!sampled) {
+        cse_exact_result r;
+        memset(&r, 0, sizeof(r));
+        rc = cse_exact_projector(*ctx, P, L, k, exact, &r);
+        if (rc == CSE_OK) *value = r.value;
+        return rc;
+    }
+    if (bins == 1) return sampled_mean(*ctx, P, L, k, exact, samples, seed, 0, value);
+    double *vals = (double *)malloc(sizeof(double) * (size_t)bins);
+    if (!vals) return CSE_ERR_NOMEM;
+    for (int b = 0; b < bins && rc == CSE_OK; b++)
+        rc = sampled_mean(*ctx, P, L, k, exact, samples, seed, (int64_t)b * samples, &vals[b]);
+    if (rc == CSE_OK) {
+        qsort(vals, (size_t)bins, sizeof(double), cmp_double);
+        *value = (bins % 2) ? vals[(bins - 1) / 2] : (vals[bins / 2] + vals[bins / 2 - 1]) / 2;
+    }
+    free(vals);
+    return rc;
+}

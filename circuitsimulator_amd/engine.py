@@ -4,6 +4,7 @@ engine, used by the tests, bench.py and anyone driving the engine from Python.
 There is no CPU path here: if libcse.so is missing or no gfx950 GPU is
 present, the calls raise.
 """
+import atexit
 import ctypes
 import os
 
@@ -22,8 +23,9 @@ P = ctypes.POINTER
 ABI_SYMBOLS = [
     "cse_create", "cse_destroy", "cse_error", "cse_device_info", "cse_inner_products",
     "cse_prep_states", "cse_random_states", "cse_exact_projector", "cse_sampled_projector",
-    "cse_batch_exact", "cse_decompose", "cse_hist_value",
+    "cse_batch_exact", "cse_decompose", "cse_hist_value", "cse_sampling_kept", "cse_projector_norm",
 ]
+ERR_NODEV = -5
 
 
 class Projector(ctypes.Structure):
@@ -93,6 +95,10 @@ def lib():
                                     P(ctypes.c_int), ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int]
         L.cse_hist_value.argtypes = [P(ctypes.c_int64), ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]
+        L.cse_sampling_kept.argtypes = [ctypes.c_int] * 7
+        L.cse_projector_norm.argtypes = [P(ctypes.c_void_p), ctypes.c_int, P(Projector), P(U64), ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         U64, P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -219,6 +225,43 @@ def decompose(t, seed, exact=0, k=0, fidbound=1e-5, rank=0, fidelity=0, forceL=0
     if rc != 0:
         raise EngineError(f"cse_decompose failed ({rc})")
     return [int(L[i]) for i in range(kk.value if not ex.value else 0)], norm.value, ex.value, kk.value
+
+
+def sampling_kept(noapprox, forceSample, exact, t, k, samples, bins):
+    """The sampling-disable rule (probability.c:162-174)."""
+    return bool(lib().cse_sampling_kept(int(bool(noapprox)), int(bool(forceSample)), int(bool(exact)), int(t),
+                                        int(k), int(samples), int(bins)))
+
+
+# one engine context per device for the projector norms, created by the
+# library on the first call that needs the GPU and kept for the process
+_norm_ctx = {}
+
+
+def _destroy_norm_contexts():
+    for h in _norm_ctx.values():
+        if h:
+            lib().cse_destroy(h)
+    _norm_ctx.clear()
+
+
+atexit.register(_destroy_norm_contexts)
+
+
+def projector_norm(proj, L, k, exact, norm, sampled, samples=0, bins=1, seed=0, device=0):
+    """||P|H^t>||^2 as the backend computes it (cse_projector_norm):
+    exact, or the median over bins of sampled means."""
+    h = _norm_ctx.setdefault(device, ctypes.c_void_p())
+    Larr = (U64 * NMAX)(*(list(L) + [0] * (NMAX - len(L))))
+    v = ctypes.c_double()
+    rc = lib().cse_projector_norm(ctypes.byref(h), device, ctypes.byref(proj), Larr, int(k), int(bool(exact)),
+                                  float(norm), int(bool(sampled)), int(samples), int(bins), seed & (2 ** 64 - 1),
+                                  ctypes.byref(v))
+    if rc == ERR_NODEV:
+        raise EngineError(f"no usable gfx950 device {device} for the engine")
+    if rc != 0:
+        raise EngineError(f"engine error {rc}: {lib().cse_error(h).decode()}")
+    return v.value
 
 
 def hist_value(hist, pmin=-128):

@@ -22,6 +22,7 @@ extern "C" {
 #define CSE_ERR_HIP (-2)
 #define CSE_ERR_NOMEM (-3)
 #define CSE_ERR_RNG (-4)
+#define CSE_ERR_NODEV (-5)   /* no usable gfx950 device for a context */
 
 #define CSE_NMAX 64
 
@@ -171,6 +172,26 @@ int cse_decompose(int t, uint64_t seed, uint64_t *L, double *norm, int *exact, i
 
 /* exact value of a histogram as FP64 (double-double internally) */
 void cse_hist_value(const int64_t *hist, int pmin, double *re, double *im);
+
+/*
+ * The sampling-disable rule of probability.c:162-174: returns 1 when the
+ * projector norms are to be sampled, 0 when they are computed exactly
+ * (noapprox, or samples * bins * 2 > chi - 1 without forceSample).
+ */
+int cse_sampling_kept(int noapprox, int forceSample, int exact, int t, int k, int samples, int bins);
+
+/*
+ * ||P |H^t>||^2 as the backend computes it for G' and H'
+ * (probability.c:185-205): exactProjector (innerprod.c:148-199) when
+ * sampled == 0, else multiSampledProjector (innerprod.c:23-84) -- the median
+ * over bins of the mean over samples, bin b drawing the random states
+ * [b * samples, (b + 1) * samples) of the Philox stream keyed by seed.  The
+ * empty projector gives norm^2 and t = 0 the Clifford closed form, without
+ * the GPU.  *ctx is created on device when first needed (CSE_ERR_NODEV if
+ * that fails); the caller destroys it.
+ */
+int cse_projector_norm(cse_ctx **ctx, int device, const cse_projector *P, const uint64_t *L, int k, int exact,
+                       double norm, int sampled, int samples, int bins, uint64_t seed, double *value);
 
 #ifdef __cplusplus
 }
