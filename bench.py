@@ -166,6 +166,65 @@ def batch_config(args):
     }
 
 
+def circuit_config(args):
+    """C1: probability of {0:0, 1:0} for the seeded 8-qubit circuit of 40
+    {H, S, CX} gates and 6 T gates, exact decomposition and exact pair sums
+    (noapprox), through the whole front end (gadgetize, truncate) and the
+    engine.  value = probabilities/s."""
+    from circuitsimulator_amd.probability import probability
+    from circuitsimulator_amd.workloads import c1_circuit
+    circ, y = c1_circuit()
+    cfg = {"noapprox": True, "exact": True, "quiet": True, "x": "", "y": y, "seed": 1}
+    for _ in range(args.warmup):
+        probability(circ, {0: 0, 1: 0}, config=dict(cfg))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p = probability(circ, {0: 0, 1: 0}, config=dict(cfg))
+    dt = time.perf_counter() - t0
+    return {
+        "metric": "probability evaluations/s (front end + engine), C1 circuit, exact",
+        "value": args.steps / dt, "unit": "probabilities/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1000.0 / args.steps, "higher_is_better": True,
+        "dtype": "int64", "data": "synthetic (seeded circuit)",
+        "config": {"workload": "C1: n=8, 40 {H,S,CX} + 6 T, measure {0:0, 1:0}, t=6, chi=8, "
+                               "36 pair terms per projector", "seed": 1001},
+        "probability": p,
+    }
+
+
+def shots_config(args):
+    """C5: qubit-by-qubit sampling of 8 qubits of a seeded 40-qubit circuit
+    of 400 {H, S, CX} gates and 40 T gates (t = 40), 1024 shots, |L> with
+    k = 8 (chi = 256; 2000 samples per bin exceed chi, so the reference's
+    rule evaluates each norm exactly), all shots advanced together.  value =
+    shots/s."""
+    import numpy as np
+
+    from circuitsimulator_amd.sample import sampleShots
+    from circuitsimulator_amd.workloads import c1_circuit
+    circ, _ = c1_circuit(seed=1005, n=40, gates=400, ts=40)
+    shots = args.shots
+    cfg = {"exact": False, "k": 8, "quiet": True}
+    if args.warmup:
+        sampleShots(circ, {}, list(range(8)), 8, config=dict(cfg, rng=np.random.default_rng(7)))
+    t0 = time.perf_counter()
+    out = sampleShots(circ, {}, list(range(8)), shots, config=dict(cfg, rng=np.random.default_rng(1005)))
+    dt = time.perf_counter() - t0
+    freq = {}
+    for o in out:
+        freq[o] = freq.get(o, 0) + 1
+    return {
+        "metric": "sampled shots/s, 8 qubits of a 40-qubit Clifford+T circuit (t=40)",
+        "value": shots / dt, "unit": "shots/s", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
+        "ms_per_step": dt * 1000.0, "higher_is_better": True, "dtype": "int64",
+        "data": "synthetic (seeded circuit)",
+        "config": {"workload": "C5: n=40, 400 {H,S,CX} + 40 T, sample qubits 0..7, |L> k=8, "
+                               f"{shots} shots", "seed": 1005, "shots": shots, "k": 8},
+        "inner_products": shots * 8 * 2 * (256 * 257 // 2),
+        "distinct_outcomes": len(freq),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,8 +234,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--plain-pairs", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--config", choices=["c4", "c2", "c3"], default="c4",
-                    help="c4: the headline norm estimation (default); c2, c3: the other GPU configurations")
+    ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5"], default="c4",
+                    help="c4: the headline norm estimation (default); c1, c2, c3, c5: the other configurations")
+    ap.add_argument("--shots", type=int, default=1024, help="c5: number of shots")
     ap.add_argument("--instances", type=int, default=4096, help="c3: number of instances")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -186,7 +246,8 @@ def main():
         sys.exit(2)
 
     if args.config != "c4":
-        out = exact_pairs_config(args) if args.config == "c2" else batch_config(args)
+        out = {"c1": circuit_config, "c2": exact_pairs_config, "c3": batch_config,
+               "c5": shots_config}[args.config](args)
         print(json.dumps(out), flush=True)
         return
 

@@ -105,3 +105,37 @@ def uneven_instances(count, seed=1003, tmin=8, tmax=48, kmin=2, kmax=14):
         out.append(inst)
         specs.append((t, r, k, ph, xs, zs, L))
     return out, specs
+
+
+def c1_circuit(seed=1001, n=8, gates=40, ts=6):
+    """C1: a compiled n-qubit circuit of `gates` gates drawn uniformly from
+    {H, S, CX} (CX on an ordered pair of distinct qubits, uniform), then `ts`
+    T gates inserted at uniform slots on uniform qubits, redrawn until no
+    qubit has two T gates in a row; and the postselection y, fixed by the
+    seed.  Returns (circuit text, y string)."""
+    from .compile.gadgetize import countY
+    rng = np.random.default_rng(seed)
+    while True:
+        lines = []
+        for _ in range(gates):
+            g = int(rng.integers(0, 3))
+            row = ["_"] * n
+            if g == 2:
+                c, tq = rng.choice(n, size=2, replace=False)
+                row[c], row[tq] = "C", "X"
+            else:
+                row[int(rng.integers(0, n))] = "HS"[g]
+            lines.append("".join(row))
+        for _ in range(ts):
+            at = int(rng.integers(0, len(lines) + 1))
+            row = ["_"] * n
+            row[int(rng.integers(0, n))] = "T"
+            lines.insert(at, "".join(row))
+        circ = "\n".join(lines) + "\n"
+        try:
+            countY(circ)
+        except SyntaxError:
+            continue
+        y = "".join(str(int(b)) for b in rng.integers(0, 2, size=ts))
+        return circ, y
+

@@ -257,7 +257,75 @@ def test_ht_stack_known_answers_dense():
         assert got == pytest.approx(exact, abs=1e-12)
 
 
+def c1_projectors():
+    from circuitsimulator_amd.workloads import c1_circuit
+    circ, y = c1_circuit()
+    G, H, n, t = projectors.projectors(circ, {0: 0, 1: 0}, x="", y=y)
+    Gp, u = projectors.truncate(n, G)
+    Hp, v = projectors.truncate(n, H)
+    return circ, y, Gp, Hp, u, v, t
+
+
+def masks(P):
+    ph, xs, zs = P
+    return ([int(p) for p in ph], [sum(int(b) << j for j, b in enumerate(x)) for x in xs],
+            [sum(int(b) << j for j, b in enumerate(z)) for z in zs])
+
+
+def test_c1_probability_by_oracle_and_dense():
+    # C1: the exact pair sums of the oracle (innerprod.c:148-261 restated)
+    # and the dense norms give the brute-force probability
+    from gpu_util import oracle_exact_terms, term_value
+    circ, y, Gp, Hp, u, v, t = c1_projectors()
+    assert t == 6 and len(circ.splitlines()) == 46
+    want = sim.conditional_probability(circ, {0: 0, 1: 0}, {})
+    dense = 2.0 ** (v - u) * sim.magic_norm2(Gp, t) / sim.magic_norm2(Hp, t)
+    assert dense == pytest.approx(want, abs=1e-12)
+
+    def oracle_norm(P):
+        if len(P[0]) == 0:
+            return 2.0 ** (t // 2) * (1 if t % 2 == 0 else 4 * np.cos(np.pi / 8) ** 2)
+        tot = 0
+        for pt, s, i, j in oracle_exact_terms(t, 0, [], 1, *masks(P)):
+            if pt:
+                val = term_value(pt, s)
+                tot += val if i == j else 2 * val.real
+        return abs(tot)
+
+    got = 2.0 ** (v - u) * oracle_norm(Gp) / oracle_norm(Hp)
+    assert got == pytest.approx(want, abs=1e-12)
+
+
 # ------------------------------------------------------------------ GPU
+@pytest.mark.gpu
+def test_c1_on_engine():
+    # probability() end to end, and every pair term of G' and H' bit-exact
+    from circuitsimulator_amd import Engine
+    from circuitsimulator_amd.engine import Instance, Projector
+    from circuitsimulator_amd.probability import probability
+    from circuitsimulator_amd.workloads import c1_circuit
+    from gpu_util import oracle_exact_terms
+    circ, y, Gp, Hp, u, v, t = c1_projectors()
+    want = sim.conditional_probability(circ, {0: 0, 1: 0}, {})
+    got = probability(circ, {0: 0, 1: 0}, config={"noapprox": True, "exact": True, "quiet": True, "y": y,
+                                                   "x": "", "seed": 1})
+    assert got == pytest.approx(want, abs=1e-12)
+    with Engine(0) as E:
+        for P in (Gp, Hp):
+            if not P[0]:
+                continue
+            ph, xs, zs = masks(P)
+            inst = Instance()
+            inst.P = Projector.make(ph, xs, zs, t)
+            inst.exact = 1
+            inst.k = 0
+            nterms = 8 * 9 // 2
+            res, terms, _ = E.batch_exact([inst], want_terms=True, nterms=nterms)
+            ref = [pt for pt, s, i, j in oracle_exact_terms(t, 0, [], 1, ph, xs, zs)]
+            assert list(terms) == ref
+
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("seed", range(4))
 def test_probability_exact_on_engine(seed):
@@ -314,3 +382,92 @@ def test_sample_qubits_on_engine():
         out = sampleQubits(circ, {}, [0, 1], config={"quiet": True, "noapprox": True, "exact": True, "seed": s,
                                                      "rng": np.random.default_rng(s)})
         assert out in ("00", "11")
+
+
+# ------------------------------------------------------------ batched shots
+def test_batched_truncation_matches_truncate():
+    from circuitsimulator_amd.sample import _products, _shot_projector
+    rng = np.random.default_rng(12)
+    for trial in range(8):
+        n = int(rng.integers(3, 6))
+        while True:
+            circ = random_clifford_t(rng, n, int(rng.integers(8, 20)), int(rng.integers(1, 6)))
+            if valid_ts(circ):
+                break
+        t = gadgetize.countY(circ)[0]
+        y = [int(b) for b in rng.integers(0, 2, size=t)]
+        order = [int(q) for q in rng.permutation(n)[: int(rng.integers(1, n + 1))]]
+        G = gadgetize.gadgetize(circ, {q: 0 for q in order}, y)
+        for m in range(1, len(order) + 1):
+            rows = (G[0][:m] + G[0][len(order):], G[1][:m] + G[1][len(order):], G[2][:m] + G[2][len(order):])
+            prods = _products(n, rows)
+            for _ in range(3):
+                vals = [int(b) for b in rng.integers(0, 2, size=m)]
+                got = _shot_projector(prods, np.array(vals + [0] * (len(rows[0]) - m)))
+                Gd = gadgetize.gadgetize(circ, {q: vals[i] for i, q in enumerate(order[:m])}, y)
+                Gp, u = projectors.truncate(n, Gd)
+                assert u == prods[4]
+                neg = any(p == 2 and not x.any() and not z.any() for p, x, z in zip(*Gp))
+                if neg:
+                    assert got is None
+                    continue
+                want = masks(Gp)
+                assert got == (want[0], want[1], want[2])
+
+
+def test_batched_shots_clifford_on_cpu():
+    # t = 0 uses the closed form: no GPU needed.  GHZ: the joint chain gives
+    # only 000 and 111; the reference's chain (sample.py:54,84) divides the
+    # third joint probability by the second qubit's conditional one (1), so
+    # after 00 the third bit is 0 or 1 with probability 1/2 each
+    from circuitsimulator_amd.sample import sampleQubits, sampleShots
+    circ = "H__\nCX_\n_CX\n"
+    out = sampleShots(circ, {}, [0, 1, 2], 400, config={"noapprox": True, "chain": "joint",
+                                                        "rng": np.random.default_rng(2)})
+    assert set(out) == {"000", "111"}
+    out = sampleShots(circ, {}, [0, 1, 2], 400, config={"noapprox": True, "rng": np.random.default_rng(2)})
+    assert set(out) == {"000", "001", "111"}
+    assert abs(out.count("111") / 400 - 0.5) < 0.1
+    seq = [sampleQubits(circ, {}, [0, 1, 2], config={"noapprox": True, "rng": np.random.default_rng(s)})
+           for s in range(60)]
+    assert set(seq) <= {"000", "001", "111"}
+
+
+@pytest.mark.gpu
+def test_batched_shots_ghz_with_t_phases():
+    from circuitsimulator_amd.sample import sampleShots
+    circ = "H__\nT__\nCX_\n_CX\n__T\n_S_\n"
+    out = sampleShots(circ, {}, [0, 1, 2], 256, config={"exact": True, "chain": "joint",
+                                                        "rng": np.random.default_rng(4)})
+    assert set(out) == {"000", "111"}
+
+
+@pytest.mark.gpu
+def test_batched_shots_two_qubit_frequencies():
+    # for two sampled qubits the draws follow the true distribution
+    from circuitsimulator_amd.sample import sampleShots
+    rng = np.random.default_rng(8)
+    while True:
+        circ = random_clifford_t(rng, 3, 14, 4)
+        if valid_ts(circ):
+            break
+    psi, n = sim.run(circ)
+    probs = {}
+    for b0, b1 in itertools.product((0, 1), repeat=2):
+        pj = sim.project(psi, {0: b0, 2: b1})
+        probs["%d%d" % (b0, b1)] = np.vdot(pj, pj).real
+    shots = 4000
+    out = sampleShots(circ, {}, [0, 2], shots, config={"exact": True, "rng": np.random.default_rng(5)})
+    for key, p in probs.items():
+        f = out.count(key) / shots
+        assert abs(f - p) < 4 * np.sqrt(p * (1 - p) / shots) + 1e-9, (key, f, p)
+
+
+@pytest.mark.gpu
+def test_batched_shots_constraints_and_L_decomposition():
+    from circuitsimulator_amd.sample import sampleShots
+    circ = "H__\nT__\nCX_\nH__\nT__\n_CX\n__T\n__H\n"
+    out = sampleShots(circ, {1: 0}, [0, 2], 200, config={"exact": False, "k": 2, "rng": np.random.default_rng(6)})
+    assert all(len(o) == 2 and set(o) <= {"0", "1"} for o in out)
+    imp = sampleShots("X_\nT_\n", {0: 0}, [1], 10, config={"exact": True, "rng": np.random.default_rng(7)})
+    assert imp == [""] * 10
