@@ -27,7 +27,7 @@ struct cse_ctx {
     char name[128] = {0};
     char err[512] = {0};
     DevBuf s1, s2, out, recs, terms, zsum, kahan, hist, ips, queue, itemoff, inst, projs, thr, sexp, L,
-        idx, errflag, recoff;
+        idx, errflag, recoff, lsh;
     bool thr_ready = false;
     int blocks_per_cu = 8;  // persistent grid: blocks of 4 waves per CU
 };
@@ -306,7 +306,7 @@ void cse_destroy(cse_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->s1, &ctx->s2, &ctx->out, &ctx->recs, &ctx->terms, &ctx->zsum, &ctx->kahan,
                       &ctx->hist, &ctx->ips, &ctx->queue, &ctx->itemoff, &ctx->inst, &ctx->projs, &ctx->thr,
-                      &ctx->sexp, &ctx->L, &ctx->idx, &ctx->errflag, &ctx->recoff};
+                      &ctx->sexp, &ctx->L, &ctx->idx, &ctx->errflag, &ctx->recoff, &ctx->lsh};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->e0) (void)hipEventDestroy(ctx->e0);
@@ -575,6 +575,27 @@ int cse_batch_exact(cse_ctx *ctx, const cse_instance *inst, int ninst, int chunk
     }
     HIPCHK(hipSetDevice(ctx->device));
     return exact_batch(ctx, inst, ninst, chunk, res, terms_out, kernel_ms);
+}
+
+int cse_lspace_weights(cse_ctx *ctx, const uint64_t *L, int k, int64_t *hist) {
+    if (!ctx || !hist || (k > 0 && !L)) return CSE_ERR_ARG;
+    if (k < 0 || k > 40) return fail(ctx, CSE_ERR_ARG, "k=%d outside [0, 40]", k);
+    HIPCHK(hipSetDevice(ctx->device));
+    ENSURE(lsh, 8 * 65);
+    ENSURE(L, 8 * (size_t)CSE_NMAX);
+    if (k > 0) HIPCHK(hipMemcpyAsync(ctx->L.p, L, 8 * (size_t)k, hipMemcpyHostToDevice, ctx->stream));
+    int rc = zero64(ctx, ctx->lsh.p, 65);
+    if (rc) return rc;
+    // 2^(k - shift) threads, each walking 2^shift combinations
+    const int shift = k > 18 ? k - 18 : 0;
+    const uint64_t threads = 1ull << (k - shift);
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+    hipLaunchKernelGGL(k_lspace_hist, dim3(blocks), dim3(256), 0, ctx->stream, dp<uint64_t>(ctx->L), k, shift,
+                       dp<unsigned long long>(ctx->lsh));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hist, ctx->lsh.p, 8 * 65, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CSE_OK;
 }
 
 }  // extern "C"

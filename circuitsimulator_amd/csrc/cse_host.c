@@ -4,6 +4,7 @@
  * the exact accumulators, and decompose (libcirc/probability.c:307-415).
  */
 #include <math.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -222,6 +223,42 @@ int cse_gf2_rank(const uint64_t *rows, int k)
 /* ----------------------------------------------------------- decompose */
 #define CSE_DOM_L 2u
 
+/* #{i : |x_i| = w} over x_i = sum_j bit_{k-1-j}(i) L_j, walked in Gray-code
+ * order (the multiset does not depend on the order) */
+static void lspace_weights_host(const uint64_t *L, int k, int64_t *hist)
+{
+    memset(hist, 0, 65 * sizeof(int64_t));
+    uint64_t x = 0;
+    hist[0] = 1;
+    for (uint64_t s = 1; s < (1ull << k); s++) {
+        x ^= L[k - 1 - __builtin_ctzll(s)];
+        hist[__builtin_popcountll(x)]++;
+    }
+}
+
+/* Z_L = sum_x 2^-floor(|x|/2) from the weight histogram (the integer
+ * division of probability.c:396), accumulated in long double and rounded
+ * once */
+static double fidelity_sum(const uint64_t *L, int k)
+{
+    int64_t hist[65];
+    int done = 0;
+    if (k >= 20) {
+        int dev = 0;
+        const char *s = getenv("CSE_DEVICE");
+        if (s && *s) dev = atoi(s);
+        cse_ctx *ctx = NULL;
+        if (cse_create(dev, &ctx) == CSE_OK) {
+            done = cse_lspace_weights(ctx, L, k, hist) == CSE_OK;
+            cse_destroy(ctx);
+        }
+    }
+    if (!done) lspace_weights_host(L, k, hist);
+    long double z = 0;
+    for (int w = 64; w >= 0; w--) z += ldexpl((long double)hist[w], -(w / 2));
+    return (double)z;
+}
+
 int cse_decompose(int t, uint64_t seed, uint64_t *L, double *norm, int *exact, int *k, double fidbound,
                   int rank, int fidelity, int forceL, int verbose, int quiet)
 {
@@ -270,14 +307,7 @@ int cse_decompose(int t, uint64_t seed, uint64_t *L, double *norm, int *exact, i
             }
         }
         if (fidelity) {
-            Z_L = 0;
-            for (uint64_t i = 0; i < (1ull << *k); i++) {
-                uint64_t x = 0;
-                for (int j = 0; j < *k; j++)
-                    if ((i >> (*k - 1 - j)) & 1ull) x ^= L[j];
-                int hamming = __builtin_popcountll(x);
-                Z_L += pow(2, -hamming / 2); /* integer division, probability.c:396 */
-            }
+            Z_L = fidelity_sum(L, *k);
             innerProduct = pow(2, *k) * pow(v, 2 * t) / Z_L;
             if (forceK) {
                 printf("delta = 1 - <H^t|L>: %lf\n", 1 - innerProduct);

@@ -664,6 +664,38 @@ extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_b
 // dependency latency.
 extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed(SweepArgs A) { sweep_body<true>(A); }
 
+// Histogram of the Hamming weight |x| over the 2^k combinations
+// x = sum_j i_j L_j of the rows of L (decompose's Z_L, probability.c:
+// 371-410).  Thread g of the grid walks the 2^shift combinations
+// i = (g << shift) + (Gray code of s), s = 0 .. 2^shift - 1: bit b of i
+// selects row k-1-b, as in the reference's MSB-first binrep.  The grid
+// holds exactly 2^(k-shift) threads in full blocks of 256 (k - shift >= 8)
+// or one block with the surplus threads idle.
+extern "C" __global__ void __launch_bounds__(256)
+k_lspace_hist(const uint64_t *__restrict__ L, int k, int shift, unsigned long long *__restrict__ hist) {
+    __shared__ unsigned int lh[65];
+    __shared__ uint64_t rows[64];
+    const int tid = (int)threadIdx.x;
+    if (tid < 65) lh[tid] = 0;
+    if (tid < k) rows[tid] = L[tid];
+    __syncthreads();
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + (uint64_t)tid;
+    if ((g >> (k - shift)) == 0) {
+        const uint64_t base = g << shift;
+        uint64_t x = 0;
+        for (int b = shift; b < k; b++)
+            if ((base >> b) & 1ull) x ^= rows[k - 1 - b];
+        atomicAdd(&lh[__popcll(x)], 1u);
+        const uint64_t steps = 1ull << shift;
+        for (uint64_t s = 1; s < steps; s++) {
+            x ^= rows[k - 1 - __builtin_ctzll(s)];
+            atomicAdd(&lh[__popcll(x)], 1u);
+        }
+    }
+    __syncthreads();
+    if (tid < 65 && lh[tid]) atomicAdd(&hist[tid], (unsigned long long)lh[tid]);
+}
+
 extern "C" __global__ void k_zero64(unsigned long long *p, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0ull;

@@ -24,6 +24,7 @@ ABI_SYMBOLS = [
     "cse_create", "cse_destroy", "cse_error", "cse_device_info", "cse_inner_products",
     "cse_prep_states", "cse_random_states", "cse_exact_projector", "cse_sampled_projector",
     "cse_batch_exact", "cse_decompose", "cse_hist_value", "cse_sampling_kept", "cse_projector_norm",
+    "cse_lspace_weights",
 ]
 ERR_NODEV = -5
 
@@ -96,6 +97,7 @@ def lib():
                                     ctypes.c_int, ctypes.c_int]
         L.cse_hist_value.argtypes = [P(ctypes.c_int64), ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]
         L.cse_sampling_kept.argtypes = [ctypes.c_int] * 7
+        L.cse_lspace_weights.argtypes = [ctypes.c_void_p, P(U64), ctypes.c_int, P(ctypes.c_int64)]
         L.cse_projector_norm.argtypes = [P(ctypes.c_void_p), ctypes.c_int, P(Projector), P(U64), ctypes.c_int,
                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          U64, P(ctypes.c_double)]
@@ -203,6 +205,14 @@ class Engine:
         self._check(self._L.cse_sampled_projector(self.h, ctypes.byref(proj), _ptr(L, U64), k, int(exact),
                                                   seed, first, count, chunk, ctypes.byref(r)))
         return r, extra
+
+    def lspace_weights(self, L, k):
+        """hist[w] = number of the 2^k combinations of the rows of L with
+        Hamming weight w (65 entries)."""
+        Larr = (U64 * NMAX)(*(list(L) + [0] * (NMAX - len(L))))
+        hist = np.zeros(65, dtype=np.int64)
+        self._check(self._L.cse_lspace_weights(self.h, Larr, int(k), _ptr(hist, ctypes.c_int64)))
+        return hist
 
     def batch_exact(self, instances, chunk=0, want_terms=False, nterms=0):
         arr = (Instance * len(instances))(*instances)

@@ -165,10 +165,21 @@ int cse_batch_exact(cse_ctx *ctx, const cse_instance *inst, int ninst, int chunk
  * decompose (libcirc/probability.c:307-415): norm of the decomposition, the
  * exact/L choice and L itself (rows drawn from the seeded host RNG; rank and
  * fidelity checks as in the reference).  Returns CSE_OK; *exact and *k are
- * in/out as in the reference.
+ * in/out as in the reference.  The fidelity's Z_L sums 2^-floor(|x|/2)
+ * over the 2^k combinations from a histogram of |x| (cse_lspace_weights on
+ * the GPU named by CSE_DEVICE when k >= 20 and one is present, a Gray-code
+ * walk on the host otherwise), rounded once; the reference adds the 2^k
+ * terms one by one in FP64.
  */
 int cse_decompose(int t, uint64_t seed, uint64_t *L, double *norm, int *exact, int *k,
                   double fidbound, int rank, int fidelity, int forceL, int verbose, int quiet);
+
+/*
+ * Weights of the row space of L for decompose's fidelity Z_L
+ * (probability.c:371-410): hist[w] = #{i in [0, 2^k) : |sum_j i_j L_j| = w},
+ * w = 0..64 (65 entries), on the device.  k <= 40.
+ */
+int cse_lspace_weights(cse_ctx *ctx, const uint64_t *L, int k, int64_t *hist);
 
 /* exact value of a histogram as FP64 (double-double internally) */
 void cse_hist_value(const int64_t *hist, int pmin, double *re, double *im);

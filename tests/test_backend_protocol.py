@@ -183,3 +183,44 @@ def test_backend_sampled_mode_matches_engine():
     with Engine(0) as E:
         res, _ = E.sampled_projector(Projector.make(ph, xs, zs, t), L, k, 0, 12345, 0, 300)
     assert num == pytest.approx(res.mean, rel=1e-15)
+
+
+def _row_space_weights(L, k):
+    xs = np.zeros(1, dtype=np.uint64)
+    for row in L[:k]:
+        xs = np.concatenate([xs, xs ^ np.uint64(row)])
+    return np.bincount(np.bitwise_count(xs), minlength=65)
+
+
+def test_decompose_fidelity_is_exact():
+    # Z_L = sum over the 2^k combinations of 2^-floor(|x|/2), summed exactly
+    from fractions import Fraction
+    from circuitsimulator_amd.engine import decompose
+    for t, k, seed in [(10, 6, 1), (16, 12, 2), (24, 16, 3), (40, 18, 4)]:
+        L, norm, exact, kk = decompose(t, seed, exact=0, k=k, fidelity=1)
+        assert kk == k and not exact
+        hist = _row_space_weights(L, k)
+        zl = sum(Fraction(int(c), 2 ** (w // 2)) for w, c in enumerate(hist))
+        assert norm == pytest.approx(float((Fraction(2 ** k) * zl)) ** 0.5, rel=1e-15)
+
+
+@pytest.mark.gpu
+def test_lspace_weights_on_engine():
+    from circuitsimulator_amd import Engine
+    rng = np.random.default_rng(11)
+    with Engine(0) as E:
+        for k, t in [(0, 5), (1, 5), (7, 12), (8, 30), (13, 48), (18, 64), (19, 40), (22, 48)]:
+            L = [int(rng.integers(0, 2 ** 62)) & ((1 << t) - 1) for _ in range(k)]
+            assert list(E.lspace_weights(L, k)) == list(_row_space_weights(L, k))
+
+
+
+@pytest.mark.gpu
+def test_decompose_fidelity_device_path():
+    # k >= 20 takes the device histogram inside cse_decompose
+    from fractions import Fraction
+    from circuitsimulator_amd.engine import decompose
+    L, norm, exact, k = decompose(48, 9, exact=0, k=21, fidelity=1)
+    hist = _row_space_weights(L, 21)
+    zl = sum(Fraction(int(c), 2 ** (w // 2)) for w, c in enumerate(hist))
+    assert norm == pytest.approx(float(Fraction(2 ** 21) * zl) ** 0.5, rel=1e-15)
