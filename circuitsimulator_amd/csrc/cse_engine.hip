@@ -19,6 +19,13 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// pinned host memory: device-to-host copies into it are single DMA
+// transfers instead of chunked staging through a driver buffer
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
 struct cse_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -28,6 +35,7 @@ struct cse_ctx {
     char err[512] = {0};
     DevBuf s1, s2, out, recs, terms, zsum, kahan, hist, ips, queue, itemoff, inst, projs, thr, sexp, L,
         idx, errflag, recoff, lsh;
+    HostBuf hstage;  // per-sample sums of cse_sampled_projector
     bool thr_ready = false;
     int blocks_per_cu = 8;  // persistent grid: blocks of 4 waves per CU
 };
@@ -71,6 +79,21 @@ int ensure(cse_ctx *ctx, DevBuf &b, size_t bytes) {
         int rc_ = ensure(ctx, ctx->buf, (bytes));     \
         if (rc_) return rc_;                          \
     } while (0)
+
+int ensure_host(cse_ctx *ctx, HostBuf &b, size_t bytes) {
+    if (b.cap >= bytes) return CSE_OK;
+    if (b.p) HIPCHK(hipHostFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 4;
+    hipError_t e = hipHostMalloc(&b.p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, CSE_ERR_NOMEM, "hipHostMalloc(%zu): %s", want, hipGetErrorString(e));
+    }
+    b.cap = want;
+    return CSE_OK;
+}
 
 template <typename T>
 T *dp(DevBuf &b) { return (T *)b.p; }
@@ -309,6 +332,7 @@ void cse_destroy(cse_ctx *ctx) {
                       &ctx->sexp, &ctx->L, &ctx->idx, &ctx->errflag, &ctx->recoff, &ctx->lsh};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
+    if (ctx->hstage.p) (void)hipHostFree(ctx->hstage.p);
     if (ctx->e0) (void)hipEventDestroy(ctx->e0);
     if (ctx->e1) (void)hipEventDestroy(ctx->e1);
     if (ctx->e2) (void)hipEventDestroy(ctx->e2);
@@ -459,19 +483,26 @@ int cse_sampled_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
 
-    std::vector<long long> z(4 * (size_t)count);
-    std::vector<double> kz(2 * (size_t)count);
-    std::vector<int32_t> se((size_t)count);
-    unsigned long long nips = 0;
-    int errv = 0;
-    HIPCHK(hipMemcpyAsync(z.data(), ctx->zsum.p, 32 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(kz.data(), ctx->kahan.p, 16 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(se.data(), ctx->sexp.p, 4 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(&nips, ctx->ips.p, 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(&errv, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    // one pinned block: z (32 B/sample), kz (16), sexp (4), then ips and the error flag
+    const size_t n = (size_t)count;
+    rc = ensure_host(ctx, ctx->hstage, 52 * n + 16);
+    if (rc) return rc;
+    char *hs = (char *)ctx->hstage.p;
+    const long long *z = (const long long *)hs;
+    const double *kz = (const double *)(hs + 32 * n);
+    const int32_t *se = (const int32_t *)(hs + 48 * n);
+    unsigned long long *nipsp = (unsigned long long *)(hs + 52 * n);
+    int *errp = (int *)(hs + 52 * n + 8);
+    HIPCHK(hipMemcpyAsync(hs, ctx->zsum.p, 32 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hs + 32 * n, ctx->kahan.p, 16 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hs + 48 * n, ctx->sexp.p, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(nipsp, ctx->ips.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(errp, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     if (res->terms)
         HIPCHK(hipMemcpyAsync(res->terms, ctx->terms.p, 4 * (size_t)pl.nterms, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    const unsigned long long nips = *nipsp;
+    const int errv = *errp;
     if (errv & 1) return fail(ctx, CSE_ERR_RNG, "random state draw exceeded its shrink cap");
     if (errv & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
 
@@ -481,10 +512,10 @@ int cse_sampled_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *
     res->vanished = 0;
     for (int64_t s = 0; s < count; s++)
         if (se[s] < 0) res->vanished++;
-    cse_sampled_estimators(t, count, z.data(), se.data(), kz.data(), &res->mean, &res->mean_kahan);
-    if (res->zsum) memcpy(res->zsum, z.data(), 32 * (size_t)count);
-    if (res->sexp) memcpy(res->sexp, se.data(), 4 * (size_t)count);
-    if (res->kahan) memcpy(res->kahan, kz.data(), 16 * (size_t)count);
+    cse_sampled_estimators(t, count, z, se, kz, &res->mean, &res->mean_kahan);
+    if (res->zsum) memcpy(res->zsum, z, 32 * n);
+    if (res->sexp) memcpy(res->sexp, se, 4 * n);
+    if (res->kahan) memcpy(res->kahan, kz, 16 * n);
     return CSE_OK;
 }
 

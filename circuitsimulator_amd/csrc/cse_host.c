@@ -167,26 +167,47 @@ void cse_hist_value(const int64_t *hist, int pmin, double *re, double *im)
     *im = i.hi + i.lo;
 }
 
+/* (hi * 2^64 + lo) * 2^e in double-double; the carry of lo joins hi
+ * exactly, leaving two exact 128-bit integers to convert */
+static cse_dd dd_split_sum(__int128 hi, unsigned __int128 lo, int e)
+{
+    const __int128 top = hi + (__int128)(lo >> 64);
+    const __int128 low = (__int128)(uint64_t)lo;
+    return cse_dd_add(dd_scale(cse_dd_from_i128(top), e + 64), dd_scale(cse_dd_from_i128(low), e));
+}
+
 void cse_sampled_estimators(int t, int64_t count, const long long *z, const int32_t *sexp, const double *kz,
                             double *mean, double *mean_kahan)
 {
-    cse_dd X = {0, 0}, Y = {0, 0};
+    /* exact sums of A and B per projection exponent s (0..64): each
+     * __int128 term is split at 2^64, the high parts summed signed and the
+     * low parts unsigned (no overflow below 2^63 samples) */
+    __int128 ahi[65] = {0}, bhi[65] = {0};
+    unsigned __int128 alo[65] = {0}, blo[65] = {0};
     double kacc = 0, kcomp = 0;
     for (int64_t s = 0; s < count; s++) {
-        if (sexp[s] < 0) continue;
-        __int128 c0 = z[4 * s], c1 = z[4 * s + 1], c2 = z[4 * s + 2], c3 = z[4 * s + 3];
-        __int128 A = c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3;
-        __int128 B = c0 * c1 - c0 * c3 + c1 * c2 + c2 * c3;
-        const int e = t - sexp[s] - 64;
-        X = cse_dd_add(X, dd_scale(cse_dd_from_i128(A), e));
-        Y = cse_dd_add(Y, dd_scale(cse_dd_from_i128(B), e));
+        const int se = sexp[s];
+        if (se < 0 || se > 64) continue;
+        const __int128 c0 = z[4 * s], c1 = z[4 * s + 1], c2 = z[4 * s + 2], c3 = z[4 * s + 3];
+        const __int128 A = c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3;
+        const __int128 B = c0 * c1 - c0 * c3 + c1 * c2 + c2 * c3;
+        ahi[se] += A >> 64;
+        alo[se] += (uint64_t)A;
+        bhi[se] += B >> 64;
+        blo[se] += (uint64_t)B;
         /* FP32-Kahan path: |S|^2 of the FP32 sums, summed compensated in FP64 */
         double kr = kz[2 * s], ki = kz[2 * s + 1];
-        double v = ldexp(kr * kr + ki * ki, t - sexp[s]);
+        double v = ldexp(kr * kr + ki * ki, t - se);
         double yk = v - kcomp;
         double tk = kacc + yk;
         kcomp = (tk - kacc) - yk;
         kacc = tk;
+    }
+    cse_dd X = {0, 0}, Y = {0, 0};
+    for (int se = 0; se <= 64; se++) {
+        const int e = t - se - 64;
+        X = cse_dd_add(X, dd_split_sum(ahi[se], alo[se], e));
+        Y = cse_dd_add(Y, dd_split_sum(bhi[se], blo[se], e));
     }
     cse_dd tot = cse_dd_add(X, cse_dd_mul(Y, SQRT2));
     cse_dd inv = {1.0 / (double)count, 0};

@@ -58,3 +58,42 @@ def test_library_links_no_oracle():
     assert "liborc" not in out and "libref" not in out
     syms = subprocess.run(["nm", "-D", LIB], capture_output=True, text=True).stdout
     assert " orc_" not in syms and " ref_" not in syms
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libcse.so not built")
+def test_sampled_estimator_sum_is_exact():
+    """The per-sample estimator 2^(t-s) |S|^2, S = 2^-32 sum_w c_w omega^w
+    (innerprod.c:83,142), averaged over samples: |S|^2 2^64 = A + sqrt2 B
+    with A = sum c_w^2 and B = c0c1 - c0c3 + c1c2 + c2c3 (the cosines of
+    multiples of pi/4).  Checked against rational arithmetic on inputs whose
+    A and B cross 2^64 in both signs, spread over many exponents, with
+    vanished samples mixed in."""
+    import decimal
+    from fractions import Fraction
+    lib = ctypes.CDLL(LIB)
+    rng = __import__("numpy").random.default_rng(7)
+    t, count = 40, 3000
+    z = []
+    for s in range(count):
+        scale = int(rng.integers(0, 62))
+        z += [int(rng.integers(-(1 << 62), 1 << 62)) >> scale for _ in range(4)]
+    z[:4] = [(1 << 62) - 1, (1 << 62) - 1, (1 << 62) - 1, -(1 << 62)]
+    sexp = [int(v) for v in rng.integers(-1, 41, size=count)]
+    kz = [0.0] * (2 * count)
+    mean, mean_k = ctypes.c_double(), ctypes.c_double()
+    lib.cse_sampled_estimators(ctypes.c_int(t), ctypes.c_int64(count), (ctypes.c_longlong * len(z))(*z),
+                               (ctypes.c_int32 * count)(*sexp), (ctypes.c_double * len(kz))(*kz),
+                               ctypes.byref(mean), ctypes.byref(mean_k))
+    X = Y = Fraction(0)
+    for s in range(count):
+        if sexp[s] < 0:
+            continue
+        c0, c1, c2, c3 = z[4 * s:4 * s + 4]
+        w = Fraction(2) ** (t - sexp[s] - 64)
+        X += w * (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3)
+        Y += w * (c0 * c1 - c0 * c3 + c1 * c2 + c2 * c3)
+    decimal.getcontext().prec = 60
+    D = decimal.Decimal
+    want = (D(X.numerator) / D(X.denominator) + D(2).sqrt() * D(Y.numerator) / D(Y.denominator)) / count
+    assert abs(D(mean.value) - want) <= abs(want) * D(2) ** -52
+    assert mean_k.value == 0.0
