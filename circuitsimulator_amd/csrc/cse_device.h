@@ -268,7 +268,7 @@ DEV bool rs_random_state(RState &s, int n, const uint64_t *thr, uint64_t key, ui
 // stays symmetric, so bit x of row i is J_xi) and D_x in Z8 (always even).
 // Uniform: the active rows, h and Q.
 //
-// Packed form (PK = true, n <= 61): D_x lives in bits 61..63 of the J row,
+// Packed form (PK = true, n <= 60): D_x lives in bits 61..63 of the J row,
 // so one broadcast of row a gives both the J-row and D_a, and adding to D is
 // a 64-bit add on the row (mod 8 in the top three bits, no carry into the
 // row).  Unpacked form (PK = false): D_x in its own register.
@@ -437,113 +437,118 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
 //    solved for i = min(eta) with Sp = eta \ {i} (pk_eliminate, c = D_a/4);
 //  D_a in {0,4}, eta = 0: factor 2 (D_a = 0) or the sum vanishes (D_a = 4).
 // Column i and row i are dead after a step, so Jc keeps the stored diagonal
-// bit of row i, M may contain i, and eta (which contains i) stands for Sp.  With nA, nB, nC the numbers of steps of
-// each kind over n0 variables (n0 = nA + 2 nB + nC) the result is
-// 2^{(n0 + nC)/2} w^m, m = Q + 2 n0 - 4 nB - 2 nC + sum_B c D_i - sum_A D_a/2.
-// Register use: s56 pivot index, then D << 29 (s57 = 0 makes s[56:57] its
-// 64-bit form), s[58:59] act, s60 W = bits 29..31: m * 2^29 without the
-// last sum (acc), bit 28: the sum vanished, bits 0..27: nC; s[62:63] J_a,
-// s[64:65] eta, s[66:67] Jc.  A VALU instruction reads an SGPR that
-// v_readlane wrote only two or more instructions later.
+// bit of row i, M may contain i, and eta (which contains i) stands for Sp.
+// Variable 60 is the end row: it is always active, it is the highest, and
+// its row (J = 0, D = 4) sends the loop to the exit through the vanishing
+// branch, so no step tests for an empty mask (n <= 60).  The exit leaves its
+// index in s56: 60 at the end row, the vanishing variable's index otherwise.
+// act also keeps bit 63, so bit 63 of eta = J_a & act is D_a's top bit: in a
+// {2,6} step, (J_b ^ eta) - 2 (mod 8 in the D field) is D_b - D_a for both
+// values (D ^ 4 - 2 = D - 6), and in a constrained step it is c, so eta's
+// bit 63 adds 4 c to D on the rows of M.
+//
+// With nA6 the {2,6} steps with D_a = 6, nB, nC the steps of the other two
+// kinds and n0 variables, the result is 2^{(n0 + nC)/2} w^m with
+// m = Q + n0 - 2 nB - nC + sum_B c D_i - 2 nA6 (every variable counts 1 and
+// each step corrects its own).  W (s60) is the packed term being built:
+// bits 29..31 are m, bits 0..27 are 16 (base - n0 - nC) + 1.  Register use:
+// s56 pivot index, then D_i << 29, s[58:59] act, s[62:63] J_a, s[64:65] eta,
+// s[66:67] Jc, v40 = -2 << 29.  A VALU instruction reads an SGPR that
+// v_readlane wrote only two or more instructions later.  Every step clears
+// a bit of act below 60.
+#define CSE_END_ROW 60
+#define CSE_ELIM_LOOP                                                  \
+    "L_top%=:\n\t"                                                     \
+    "s_ff1_i32_b64 s56, s[58:59]\n\t"                                  \
+    "s_bitset0_b64 s[58:59], s56\n\t"                                  \
+    "v_readlane_b32 s62, %[jlo], s56\n\t"                              \
+    "v_readlane_b32 s63, %[jhi], s56\n\t"                              \
+    "s_and_b64 s[64:65], s[62:63], s[58:59]\n\t"                       \
+    "s_bitcmp1_b32 s63, 30\n\t"                                        \
+    "s_cbranch_scc0 L_nonA%=\n\t"                                      \
+    /* D_a in {2,6}: rows in eta take J ^= eta, D -= D_a */            \
+    "v_xor_b32 %[vt0], s64, %[jlo]\n\t"                                \
+    "v_xad_u32 %[vt1], %[jhi], s65, v40\n\t"                           \
+    "v_cndmask_b32_e64 %[jlo], %[jlo], %[vt0], s[64:65]\n\t"           \
+    "v_cndmask_b32_e64 %[jhi], %[jhi], %[vt1], s[64:65]\n\t"           \
+    "s_bitcmp1_b32 s63, 31\n\t"                                        \
+    "s_cbranch_scc0 L_top%=\n\t"                                       \
+    "s_add_u32 s60, s60, 0xc0000000\n\t"                               \
+    "s_branch L_top%=\n"                                               \
+    "L_nonA%=:\n\t"                                                    \
+    /* eta without bit 63: a constraint? */                            \
+    "s_lshl_b64 s[66:67], s[64:65], 1\n\t"                             \
+    "s_cbranch_scc0 L_C%=\n\t"                                         \
+    /* D_a in {0,4} with a constraint: pivot i = min(eta).  Row i and  \
+       column i are dead from here on, so eta itself serves as Sp. */  \
+    "s_ff1_i32_b64 s56, s[64:65]\n\t"                                  \
+    "s_bitset0_b64 s[58:59], s56\n\t"                                  \
+    "v_readlane_b32 s66, %[jlo], s56\n\t"                              \
+    "v_readlane_b32 s67, %[jhi], s56\n\t"                              \
+    "s_and_b32 s56, s67, 0xe0000000\n\t"                               \
+    "s_and_b32 s67, s67, 0x1fffffff\n\t"                               \
+    "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t"                    \
+    "s_nop 1\n\t"                                                      \
+    "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"                    \
+    /* rows in Sp: J ^= Jc, D += D_i, and D += 4 where J_xi = 1.       \
+       bitop3 0x78 is S0 ^ (S1 & S2), 0x60 is S0 & (S1 ^ S2). */       \
+    "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"         \
+    "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"         \
+    "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t"                         \
+    "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t"         \
+    "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t"                             \
+    /* lanes of M = Jc, ^ Sp when D_i in {2,6} */                      \
+    "s_bitcmp1_b32 s56, 30\n\t"                                        \
+    "s_cbranch_scc0 L_nojii%=\n\t"                                     \
+    "v_xor_b32 %[vt3], %[vt3], %[vt0]\n"                               \
+    "L_nojii%=:\n\t"                                                   \
+    /* m -= 2; z_i = c = D_a / 4: m += D_i (D += 4 on M comes from     \
+       bit 63 of Sp) */                                                \
+    "s_add_u32 s60, s60, 0xc0000000\n\t"                               \
+    "s_bitcmp1_b32 s63, 31\n\t"                                        \
+    "s_cbranch_scc0 L_noc%=\n\t"                                       \
+    "s_add_u32 s60, s60, s56\n"                                        \
+    "L_noc%=:\n\t"                                                     \
+    /* rows in M: J ^= Sp */                                           \
+    "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"         \
+    "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t"         \
+    "s_branch L_top%=\n"                                               \
+    /* D_a in {0,4} without a constraint: the sum vanishes (D_a = 4,   \
+       or the end row) or nC += 1, m -= 1 */                           \
+    "L_C%=:\n\t"                                                       \
+    "s_bitcmp1_b32 s63, 31\n\t"                                        \
+    "s_cbranch_scc1 L_end%=\n\t"                                       \
+    "s_add_u32 s60, s60, 0xdffffff0\n\t"                               \
+    "s_branch L_top%=\n"
+
+// The loop on an AState (records of n <= 60, row 60 the end row); p and m
+// as as_expsum.  The base of W's low field is 64, so p = 64 - low / 16.
 DEV int as_expsum_packed(AState &s, int &p, int &m) {
-    u64 act = uniform64(s.act);
+    const u64 act0 = uniform64(s.act);
+    u64 act = act0 | bit(CSE_END_ROW) | bit(63);
     s.act = 0;
-    if (!act) {
-        p = 0;
-        m = s.Q & 7;
-        return 1;
-    }
-    const int n0 = __popcll(act);
+    const int n0 = __popcll(act0);
     unsigned jlo = (unsigned)s.J, jhi = (unsigned)(s.J >> 32);
-    unsigned W = (unsigned)__builtin_amdgcn_readfirstlane((int)((unsigned)(s.Q + 2 * n0) << 29));
-    // in every lane: sum over the {2,6} steps of D_a << 29, plus 2^32 per
-    // {0,4} step with a constraint (the -4 nB of m)
-    u64 acc = 0;
+    unsigned W = (unsigned)__builtin_amdgcn_readfirstlane(
+        (int)(((unsigned)(s.Q + n0) << 29) + 16u * (unsigned)(64 - n0) + 1u));
     unsigned vt0, vt1, vt2, vt3;
+    int endv;
     asm volatile(
-        "s_mov_b32 s57, 0\n"
-        "L_top%=:\n\t"
-        "s_ff1_i32_b64 s56, s[58:59]\n\t"
-        "s_bitset0_b64 s[58:59], s56\n\t"
-        "v_readlane_b32 s62, %[jlo], s56\n\t"
-        "v_readlane_b32 s63, %[jhi], s56\n\t"
-        "s_bitcmp1_b32 s63, 30\n\t"
-        "s_cbranch_scc0 L_nonA%=\n\t"
-        // D_a in {2,6}
-        "s_and_b64 s[64:65], s[62:63], s[58:59]\n\t"
-        "s_and_b32 s56, s63, 0xe0000000\n\t"
-        "v_xor_b32 %[vt0], s64, %[jlo]\n\t"
-        "v_xor_b32 %[vt1], s65, %[jhi]\n\t"
-        "v_subrev_u32 %[vt1], s56, %[vt1]\n\t"
-        "v_cndmask_b32_e64 %[jlo], %[jlo], %[vt0], s[64:65]\n\t"
-        "v_cndmask_b32_e64 %[jhi], %[jhi], %[vt1], s[64:65]\n\t"
-        "v_lshl_add_u64 %[acc], s[56:57], 0, %[acc]\n\t"
-        "s_cmp_lg_u64 s[58:59], 0\n\t"
-        "s_cbranch_scc1 L_top%=\n\t"
-        "s_branch L_end%=\n"
-        "L_nonA%=:\n\t"
-        "s_and_b64 s[64:65], s[62:63], s[58:59]\n\t"
-        "s_cbranch_scc0 L_C%=\n\t"
-        // D_a in {0,4} with a constraint: pivot i = min(eta).  Row i and
-        // column i are dead from here on, so eta itself serves as Sp.
-        "s_ff1_i32_b64 s56, s[64:65]\n\t"
-        "s_bitset0_b64 s[58:59], s56\n\t"
-        "v_readlane_b32 s66, %[jlo], s56\n\t"
-        "v_readlane_b32 s67, %[jhi], s56\n\t"
-        "s_and_b32 s56, s67, 0xe0000000\n\t"
-        "s_and_b32 s67, s67, 0x1fffffff\n\t"
-        "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t"
-        "s_nop 1\n\t"
-        "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"
-        // rows in Sp: J ^= Jc, D += D_i, and D += 4 where J_xi = 1.
-        // bitop3 0x78 is S0 ^ (S1 & S2), 0x60 is S0 & (S1 ^ S2).
-        "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"
-        "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t"
-        "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t"
-        "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t"
-        // lanes of M = Jc, ^ Sp when D_i in {2,6}
-        "s_bitcmp1_b32 s56, 30\n\t"
-        "s_cbranch_scc0 L_nojii%=\n\t"
-        "v_xor_b32 %[vt3], %[vt3], %[vt0]\n"
-        "L_nojii%=:\n\t"
-        // m -= 4 (acc += 2^32); z_i = c = D_a / 4: Q += D_i and D += 4 on M
-        // (folded into Sp.hi)
-        "v_lshl_add_u64 %[acc], %[four], 0, %[acc]\n\t"
-        "s_bitcmp1_b32 s63, 31\n\t"
-        "s_cbranch_scc0 L_noc%=\n\t"
-        "s_or_b32 s65, s65, 0x80000000\n\t"
-        "s_add_u32 s60, s60, s56\n"
-        "L_noc%=:\n\t"
-        // rows in M: J ^= Sp
-        "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t"
-        "s_cmp_lg_u64 s[58:59], 0\n\t"
-        "s_cbranch_scc1 L_top%=\n\t"
-        "s_branch L_end%=\n"
-        // D_a in {0,4} without a constraint: nC += 1, m -= 2
-        "L_C%=:\n\t"
-        "s_add_u32 s60, s60, 0xc0000001\n\t"
-        "s_bitcmp1_b32 s63, 31\n\t"
-        "s_cbranch_scc1 L_zero%=\n\t"
-        "s_cmp_lg_u64 s[58:59], 0\n\t"
-        "s_cbranch_scc1 L_top%=\n\t"
-        "s_branch L_end%=\n"
-        "L_zero%=:\n\t"
-        "s_or_b32 s60, s60, 0x10000000\n"
-        "L_end%=:\n"
-        : "+{s[58:59]}"(act), "+{s60}"(W), [jlo] "+v"(jlo), [jhi] "+v"(jhi), [acc] "+v"(acc),
-          [vt0] "=&v"(vt0), [vt1] "=&v"(vt1), [vt2] "=&v"(vt2), [vt3] "=&v"(vt3)
-        : [four] "v"(1ull << 32)
-        : "s56", "s57", "s62", "s63", "s64", "s65", "s66", "s67", "scc");
-    if (W & 0x10000000u) {
+        "v_mov_b32 v40, 0xc0000000\n"
+        CSE_ELIM_LOOP
+        "L_end%=:\n\t"
+        "s_mov_b32 %[endv], s56\n\t"
+        : "+{s[58:59]}"(act), "+{s60}"(W), [jlo] "+v"(jlo), [jhi] "+v"(jhi), [vt0] "=&v"(vt0),
+          [vt1] "=&v"(vt1), [vt2] "=&v"(vt2), [vt3] "=&v"(vt3), [endv] "=&s"(endv)
+        :
+        : "s56", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+    if (endv != CSE_END_ROW) {
         p = 0;
         m = 0;
         return 0;
     }
-    const unsigned half = (unsigned)(uniform64(acc) >> 30);
-    p = n0 + (int)(W & 0x0fffffffu);
-    m = (int)(((W >> 29) - half) & 7u);
+    p = 64 - (int)((W & 0x0fffffffu) >> 4);
+    m = (int)(W >> 29);
     return 1;
 }
 

@@ -247,7 +247,7 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     A.err = dp<int>(ctx->errflag);
     const int grid = sweep_grid(ctx, pl.nitems);
     bool packed = true;
-    for (const InstDesc &d : pl.inst) packed = packed && d.n <= 61;
+    for (const InstDesc &d : pl.inst) packed = packed && d.n <= CSE_END_ROW;
     if (packed) hipLaunchKernelGGL(k_sweep_packed, dim3(grid), dim3(256), 0, ctx->stream, A);
     else hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(256), 0, ctx->stream, A);
     HIPCHK(hipGetLastError());

@@ -378,6 +378,184 @@ DEV int ip_phiL(const RecView &rv, u64 lowx, u64 xj, u64 nonact) {
     return pack_term(eps, rv.k + __popcll(xj) - pp, mm);
 }
 
+// Wave-uniform constants of one |L> record for phiL_batch: the pivot set
+// with bits 60 (the loop's end row) and 63 (kept in act by the loop), its
+// part fixed to 1 when outside supp(x_j) (act & h), the D bit planes, the
+// non-pivot columns (at most two), their column ballots over the pivot rows
+// (col0 for the lowest, col1 for the other) and h bits (hq bit 0, bit 1),
+// Q + 6 (the two extra bits of act count in n0), and 16 k + 1 with k the
+// record's dimension.
+struct PhiLConst {
+    u64 actR, actH, D1, D2, nonact, col0, col1;
+    unsigned hq, Q, K16;
+};
+
+// nb terms <phi_j | theta'> for j = jnext - 1 .. jnext + nb - 2, packed, term
+// c in lane c of the result; x_j (which carries bits 60 and 63) advances by
+// flip[ctz(j + 1)] after every term (lane b of flip holds the XOR of rows
+// k-1 .. k-1-b of L).  One asm block computes what ip_phiL<true> computes,
+// per term:
+//  * fixation of the pivot variables outside supp(x_j): B = act & h & ~x_j;
+//    Q += sum_B D + 4 #edges(J inside B) (edges from the parity of J_x & B
+//    below x, one ballot), D_x += 4 |J_x & B| (bit 63 of row x);
+//  * the non-pivot coordinates outside supp(x_j), x_q = 0: with
+//    S = col_q & act the constraint S . z = h_q + |col_q & B| mod 2 is an
+//    empty sum (S = 0, right side 1), nothing (S = 0, right side 0), or an
+//    elimination of min(S) as a constrained step whose variable counts
+//    nothing (W += -1 in m, +1 in the exponent field); a second column then
+//    sees G_b ^= G_i on rows b in S, i.e. col1 ^= S and h_q1 ^= c when
+//    G_iq1 = 1;
+//  * the elimination loop, W starting at m = Q + n0 and 16 (k + |x_j| - n0)
+//    + 1, so that W's low field ends as the packed exponent;
+//  * packing: term = (W & 0x0fffffff) + 2 (W >> 29), 0 for a vanishing sum.
+// Fixed registers besides the loop's: m0 the term index (the lane
+// v_writelane writes; s47 keeps the caller's m0), s46 j + 1, s[44:45] B,
+// s[48:49] and s50 the column and h bit of the restriction at hand, s51 the
+// constraint's right side, s[52:53] the edge ballot, s54 a temporary,
+// s[62:63] the pending non-pivot columns before the loop.
+DEV int phiL_batch(u64 &xj, unsigned jnext, unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo,
+                   unsigned lxhi, unsigned flo, unsigned fhi, const PhiLConst &C) {
+    int parked;
+    unsigned jlo, jhi, vt0, vt1, vt2, vt3;
+    asm volatile(
+        "v_mov_b32 %[parked], 0\n\t"
+        "v_mov_b32 v40, 0xc0000000\n\t"
+        "s_mov_b32 s47, m0\n\t"
+        "s_mov_b32 m0, 0\n\t"
+        "s_mov_b32 s46, %[jnext]\n"
+        "L_ip%=:\n\t"
+        // fixation
+        "s_andn2_b64 s[44:45], %[actH], %[xj]\n\t"
+        "s_and_b64 s[58:59], %[actR], %[xj]\n\t"
+        "v_and_b32 %[vt0], s44, %[jlo0]\n\t"
+        "v_and_b32 %[vt1], s45, %[jhi0]\n\t"
+        "v_and_b32 %[vt3], %[vt1], %[lxhi]\n\t"
+        // (S0 & S1) ^ S2: the parity word of J_x & B below x
+        "v_bitop3_b32 %[vt2], %[vt0], %[lxlo], %[vt3] bitop3:0x6a\n\t"
+        "v_xor_b32 %[vt3], %[vt0], %[vt1]\n\t"
+        "v_bcnt_u32_b32 %[vt2], %[vt2], 0\n\t"
+        "v_bcnt_u32_b32 %[vt3], %[vt3], 0\n\t"
+        "v_and_b32 %[vt2], 1, %[vt2]\n\t"
+        "v_cmp_ne_u32_e64 s[52:53], 0, %[vt2]\n\t"
+        "v_mov_b32 %[jlo], %[jlo0]\n\t"
+        "v_lshl_add_u32 %[jhi], %[vt3], 31, %[jhi0]\n\t"
+        "s_and_b64 s[62:63], %[D1], s[44:45]\n\t"
+        "s_bcnt1_i32_b64 s51, s[62:63]\n\t"
+        "s_bcnt1_i32_b64 s50, s[58:59]\n\t"
+        "s_lshl1_add_u32 s51, s51, %[Q]\n\t"
+        "s_add_u32 s51, s51, s50\n\t"
+        "s_xor_b64 s[52:53], s[52:53], %[D2]\n\t"
+        "s_and_b64 s[52:53], s[52:53], s[44:45]\n\t"
+        "s_bcnt1_i32_b64 s54, s[52:53]\n\t"
+        "s_lshl2_add_u32 s51, s54, s51\n\t"
+        "s_bcnt1_i32_b64 s54, %[xj]\n\t"
+        "s_sub_u32 s54, s54, s50\n\t"
+        "s_lshl4_add_u32 s54, s54, %[K16]\n\t"
+        "s_lshl_b32 s51, s51, 29\n\t"
+        "s_add_u32 s60, s51, s54\n\t"
+        "s_andn2_b64 s[62:63], %[nonact], %[xj]\n\t"
+        "s_cbranch_scc1 L_rs%=\n"
+        "L_elim%=:\n"
+        CSE_ELIM_LOOP
+        "L_end%=:\n\t"
+        "s_cmp_eq_u32 s56, 60\n\t"
+        "s_cbranch_scc0 L_zero%=\n\t"
+        "s_lshr_b32 s51, s60, 29\n\t"
+        "s_and_b32 s60, s60, 0x0fffffff\n\t"
+        "s_lshl1_add_u32 s60, s51, s60\n\t"
+        "s_branch L_emit%=\n"
+        "L_zero%=:\n\t"
+        "s_mov_b32 s60, 0\n"
+        "L_emit%=:\n\t"
+        "v_writelane_b32 %[parked], s60, m0\n\t"
+        "s_ff1_i32_b32 s51, s46\n\t"
+        "s_add_u32 s46, s46, 1\n\t"
+        "v_readlane_b32 s62, %[flo], s51\n\t"
+        "v_readlane_b32 s63, %[fhi], s51\n\t"
+        "s_add_u32 m0, m0, 1\n\t"
+        "s_xor_b64 %[xj], %[xj], s[62:63]\n\t"
+        "s_cmp_lt_u32 m0, %[nb]\n\t"
+        "s_cbranch_scc1 L_ip%=\n\t"
+        "s_branch L_done%=\n"
+        // restrictions: s[62:63] = the non-pivot columns outside supp(x_j)
+        "L_rs%=:\n\t"
+        "s_ff1_i32_b64 s51, %[nonact]\n\t"
+        "s_bitcmp1_b64 s[62:63], s51\n\t"
+        "s_cbranch_scc0 L_rs1%=\n\t"
+        "s_mov_b64 s[48:49], %[col0]\n\t"
+        "s_and_b32 s50, %[hq], 1\n\t"
+        "s_bitset0_b64 s[62:63], s51\n\t"
+        "s_branch L_rdo%=\n"
+        "L_rs1%=:\n\t"
+        "s_mov_b64 s[48:49], %[col1]\n\t"
+        "s_lshr_b32 s50, %[hq], 1\n\t"
+        "s_mov_b64 s[62:63], 0\n"
+        "L_rdo%=:\n\t"
+        "s_and_b64 s[64:65], s[48:49], s[58:59]\n\t"
+        "s_and_b64 s[52:53], s[48:49], s[44:45]\n\t"
+        "s_bcnt1_i32_b64 s51, s[52:53]\n\t"
+        "s_xor_b32 s51, s51, s50\n\t"
+        "s_cmp_lg_u64 s[64:65], 0\n\t"
+        "s_cbranch_scc0 L_rempty%=\n\t"
+        "s_ff1_i32_b64 s56, s[64:65]\n\t"
+        "s_bitset0_b64 s[58:59], s56\n\t"
+        "v_readlane_b32 s66, %[jlo], s56\n\t"
+        "v_readlane_b32 s67, %[jhi], s56\n\t"
+        // the second column after this elimination
+        "s_bitcmp1_b64 %[col1], s56\n\t"
+        "s_cselect_b64 s[52:53], s[64:65], 0\n\t"
+        "s_cselect_b32 s54, s51, 0\n\t"
+        "s_xor_b64 s[48:49], %[col1], s[52:53]\n\t"
+        "s_lshr_b32 s50, %[hq], 1\n\t"
+        "s_xor_b32 s50, s50, s54\n\t"
+        // elimination of i over S with c = the constraint's right side
+        "s_and_b32 s56, s67, 0xe0000000\n\t"
+        "s_and_b32 s67, s67, 0x1fffffff\n\t"
+        "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"
+        "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"
+        "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"
+        "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t"
+        "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t"
+        "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t"
+        "s_bitcmp1_b32 s56, 30\n\t"
+        "s_cbranch_scc0 L_rnojii%=\n\t"
+        "v_xor_b32 %[vt3], %[vt3], %[vt0]\n"
+        "L_rnojii%=:\n\t"
+        "s_add_u32 s60, s60, 0xe0000010\n\t"
+        "s_bitcmp1_b32 s51, 0\n\t"
+        "s_cbranch_scc0 L_rnoc%=\n\t"
+        "s_or_b32 s65, s65, 0x80000000\n\t"
+        "s_add_u32 s60, s60, s56\n"
+        "L_rnoc%=:\n\t"
+        "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"
+        "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n"
+        "L_rnext%=:\n\t"
+        "s_cmp_lg_u64 s[62:63], 0\n\t"
+        "s_cbranch_scc0 L_elim%=\n\t"
+        "s_mov_b64 s[62:63], 0\n\t"
+        "s_branch L_rdo%=\n"
+        "L_rempty%=:\n\t"
+        "s_bitcmp1_b32 s51, 0\n\t"
+        "s_cbranch_scc1 L_zero%=\n\t"
+        "s_mov_b64 s[48:49], %[col1]\n\t"
+        "s_lshr_b32 s50, %[hq], 1\n\t"
+        "s_branch L_rnext%=\n"
+        "L_done%=:\n\t"
+        "s_mov_b32 m0, s47\n\t"
+        "s_nop 1\n\t"
+        : [parked] "=&v"(parked), [xj] "+s"(xj), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0),
+          [vt1] "=&v"(vt1), [vt2] "=&v"(vt2), [vt3] "=&v"(vt3)
+        : [jnext] "s"(jnext), [nb] "s"(nb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [lxlo] "v"(lxlo),
+          [lxhi] "v"(lxhi), [flo] "v"(flo), [fhi] "v"(fhi), [actR] "s"(C.actR), [actH] "s"(C.actH),
+          [D1] "s"(C.D1), [D2] "s"(C.D2), [nonact] "s"(C.nonact), [col0] "s"(C.col0), [col1] "s"(C.col1),
+          [hq] "s"(C.hq), [Q] "s"(C.Q), [K16] "s"(C.K16)
+        : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
+          "s60", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+    return parked;
+}
+
 // <phi_j | theta'> for phi_j = prepH(j), packed: pair constraints
 // x_{2q} = x_{2q+1} (bit 1), couplings 4 x_{2q} x_{2q+1} (bit 0), last qubit
 // of odd t fixed to 0 on bit 1 (stateprep.c:36-81, as a coordinate-form
@@ -565,7 +743,11 @@ DEV void sweep_body(const SweepArgs &A) {
         const uint64_t *rec = A.recs + (size_t)REC_WORDS * (size_t)(D.rec_off + r);
         RecView rv;
         rv.load(rec);
-        if (PK && !rv.vanished) rv.J |= (u64)rv.d << 61;
+        if (PK && !rv.vanished) {
+            rv.J |= (u64)rv.d << 61;
+            // the end row of the elimination loop: J = 0, D = 4
+            if (x == CSE_END_ROW) rv.J = 4ull << 61;
+        }
         if (rv.vanished) {
             if (A.terms)
                 for (long long j = j0 + x; j < j1; j += 64) A.terms[tbase + (j - j0)] = 0;
@@ -608,16 +790,43 @@ DEV void sweep_body(const SweepArgs &A) {
             if (x < k) {
                 for (int b = 0; b <= x; b++) flip ^= D.L[k - 1 - b];
             }
-            for (long long jb = j0; jb < j1; jb += 64) {
-                const int nb = (int)(j1 - jb < 64 ? j1 - jb : 64);
-                int parked = 0;
-                for (int c = 0; c < nb; c++) {
-                    const int packed = ip_phiL<PK>(rv, lowx, xj, nonact);
-                    if (lane_in(bit(c))) parked = packed;
+            if (PK && __popcll(nonact) <= 2) {
+                // the whole batch in one asm block
+                PhiLConst C;
+                const u64 q0m = nonact & (0ull - nonact), q1m = nonact & ~q0m;
+                const int q0 = q0m ? __builtin_ctzll(q0m) : 0, q1 = q1m ? __builtin_ctzll(q1m) : 0;
+                C.actR = rv.act | bit(CSE_END_ROW) | bit(63);
+                C.actH = rv.act & rv.h;
+                C.D1 = rv.D1;
+                C.D2 = rv.D2;
+                C.nonact = nonact;
+                C.col0 = q0m ? (ballot((rv.G >> q0) & 1ull) & rv.act) : 0ull;
+                C.col1 = q1m ? (ballot((rv.G >> q1) & 1ull) & rv.act) : 0ull;
+                C.hq = (unsigned)((q0m ? (rv.h >> q0) & 1ull : 0ull) | (q1m ? ((rv.h >> q1) & 1ull) << 1 : 0ull));
+                C.Q = (unsigned)((rv.Q + 6) & 7);
+                C.K16 = 16u * (unsigned)rv.k + 1u;
+                const unsigned jlo0 = (unsigned)rv.J, jhi0 = (unsigned)(rv.J >> 32);
+                xj |= bit(CSE_END_ROW) | bit(63);
+                for (long long jb = j0; jb < j1; jb += 64) {
+                    const int nb = (int)(j1 - jb < 64 ? j1 - jb : 64);
                     // 0 < j + 1 <= chi <= 2^30
-                    xj ^= rl64(flip, __builtin_ctz((unsigned)(jb + c + 1)));
+                    const int parked = phiL_batch(xj, (unsigned)(jb + 1), (unsigned)nb, jlo0, jhi0,
+                                                  (unsigned)lowx, (unsigned)(lowx >> 32), (unsigned)flip,
+                                                  (unsigned)(flip >> 32), C);
+                    flush(parked, nb, jb);
                 }
-                flush(parked, nb, jb);
+            } else {
+                for (long long jb = j0; jb < j1; jb += 64) {
+                    const int nb = (int)(j1 - jb < 64 ? j1 - jb : 64);
+                    int parked = 0;
+                    for (int c = 0; c < nb; c++) {
+                        const int packed = ip_phiL<PK>(rv, lowx, xj, nonact);
+                        if (lane_in(bit(c))) parked = packed;
+                        // 0 < j + 1 <= chi <= 2^30
+                        xj ^= rl64(flip, __builtin_ctz((unsigned)(jb + c + 1)));
+                    }
+                    flush(parked, nb, jb);
+                }
             }
         }
         if (ballot(bad) && x == 0) atomicOr(A.err, 2);
@@ -658,7 +867,8 @@ DEV void sweep_body(const SweepArgs &A) {
 
 extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_body<false>(A); }
 
-// the same sweep with D packed into the J rows (every instance has n <= 61).
+// the same sweep with D packed into the J rows (every instance has n <= 60;
+// row 60 is the elimination loop's end row).
 // Eight waves per SIMD (60 VGPRs, 78 SGPRs): the elimination loop is bound by
 // instruction issue, and the extra wave hides more of its VALU -> SALU
 // dependency latency.

@@ -6,14 +6,18 @@ of K in reduced row echelon form with row a stored at its pivot column, the
 quadratic form on it, and D packed into bits 61..63 of the J rows.  For
 phi_j = prepL(j) the pivot coordinates outside supp(x_j) are fixed in one
 batch, the other zero coordinates are restrictions, and the remaining
-Gaussian sum is eliminated one variable at a time by a hand-scheduled loop.
+Gaussian sum is eliminated one variable at a time by a hand-scheduled loop
+that ends at row 60 (J = 0, D = 4).
 
-This file restates that path in Python, step for step and, for the loop,
-register for register (the W word: m * 2^29 | vanished << 28 | nC, and the
-accumulator of D_a << 29 plus 2^32 per constrained step), and checks every
-term against the oracle's innerProduct (libcirc/stabilizer/stabilizer.c:
-589-667).  It also counts the step kinds so that every branch of the loop is
-known to be exercised.
+This file restates that path in Python register for register: phiL_batch
+(the asm block of records with at most two non-pivot columns: fixation with
+one edge ballot, restrictions as constrained steps with the second column
+updated in scalar registers, the loop, packing from the W word m * 2^29 |
+16 np + 1), and the fallback of records with
+more non-pivot columns (restrictions in C++, then the same loop).  Every term
+is checked against the oracle's innerProduct (libcirc/stabilizer/
+stabilizer.c:589-667), and the step kinds are counted so that every branch
+is known to be exercised.
 """
 import collections
 
@@ -103,67 +107,146 @@ def to_record(th):
     return r
 
 
-def expsum_loop(s, count):
-    """The hand-scheduled elimination loop on packed rows J_x | D_x << 61."""
-    J = [(s.J[x] & MASK61) | ((s.d[x] & 7) << 61) for x in range(64)]
-    act = s.act
-    if not act:
-        return 1, 0, s.Q & 7
-    n0 = popc(act)
-    W = ((s.Q + 2 * n0) << 29) & U32
-    acc = 0
-    while act:
+END = 60  # the end row of the elimination loop
+TOP = 63  # kept in act: bit 63 of eta is D_a's top bit
+U64 = (1 << 64) - 1
+
+
+def elim_loop(J, act, W, count):
+    """The hand-scheduled elimination loop (CSE_ELIM_LOOP) on packed rows
+    J_x | D_x << 61, act holding the end row and bit 63; returns (s56, W):
+    s56 is the index of the variable that ended the loop (END at the end
+    row)."""
+    while True:
         a = low(act)
         act &= ~(1 << a)
         Ja = J[a]
         hi = Ja >> 32
+        eta = Ja & act                           # bit 63: D_a >> 2
         if (hi >> 30) & 1:                       # D_a in {2,6}
             count["A"] += 1
-            eta = Ja & act
-            dm = hi & 0xE0000000
             for x in bits(eta):
                 lo = (J[x] & U32) ^ (eta & U32)
-                h2 = (((J[x] >> 32) ^ (eta >> 32)) - dm) & U32
+                h2 = (((J[x] >> 32) ^ (eta >> 32)) + 0xC0000000) & U32
                 J[x] = (h2 << 32) | lo
-            acc += dm
+            if (hi >> 31) & 1:                   # D_a = 6: m -= 2
+                W = (W + 0xC0000000) & U32
             continue
-        eta = Ja & act
-        if not eta:                              # no constraint
+        if not (eta << 1) & U64:                 # no constraint
+            if (hi >> 31) & 1:                   # D_a = 4, or the end row
+                if a != END:
+                    count["vanish"] += 1
+                return a, W
             count["C"] += 1
-            W = (W + 0xC0000001) & U32
-            if (hi >> 31) & 1:
-                count["vanish"] += 1
-                W |= 0x10000000
-                break
+            W = (W + 0xDFFFFFF0) & U32
             continue
         count["B"] += 1                          # constraint eta . z = D_a/4
-        i = low(eta)
-        act &= ~(1 << i)
-        Jr = J[i]
-        dmi = (Jr >> 32) & 0xE0000000
-        Jc = Jr & MASK61
-        for x in range(64):
-            lo, h2 = J[x] & U32, J[x] >> 32
-            if (eta >> x) & 1:
-                lo ^= Jc & U32
-                h2 ^= Jc >> 32
-                h2 = (h2 + dmi + (0x80000000 if (Jc >> x) & 1 else 0)) & U32
-            J[x] = (h2 << 32) | lo
-        Mm = Jc ^ (eta if (dmi >> 30) & 1 else 0)
-        sp = eta | ((1 << 63) if (hi >> 31) & 1 else 0)
-        for x in bits(Mm):
-            J[x] ^= sp
-        acc += 1 << 32
-        if (hi >> 31) & 1:
-            W = (W + dmi) & U32
-    if W & 0x10000000:
-        return 0, 0, 0
-    return 1, n0 + (W & 0x0FFFFFFF), ((W >> 29) - (acc >> 30)) & 7
+        W = constrained_step(J, eta, (hi >> 31) & 1, W, 0xC0000000)
+        act &= ~(1 << low(eta))
 
 
-def ip_phiL(r, xj, n, count):
-    """Fixation of the pivot coordinates outside supp(x_j), restrictions on
-    the other zero coordinates, then the loop; returns (eps, p, m)."""
+def constrained_step(J, eta, c, W, wadd):
+    """Elimination of i = min(eta) over Sp = eta with z_i = c, register for
+    register: rows in Sp take J ^= Jc and D += D_i + 4 J_xi, rows in M take
+    J ^= Sp (and D += 4 when c, through bit 63 of Sp), W += wadd (+ D_i)."""
+    i = low(eta)
+    Jr = J[i]
+    s66, s67 = Jr & U32, Jr >> 32
+    s56 = s67 & 0xE0000000
+    s67 &= 0x1FFFFFFF
+    jc = (s67 << 32) | s66
+    s64, s65 = eta & U32, eta >> 32
+    for x in range(64):
+        lo, h2 = J[x] & U32, J[x] >> 32
+        if (eta >> x) & 1:
+            lo ^= s66
+            h2 ^= s67
+            h2 = (h2 + (s56 ^ (0x80000000 if (jc >> x) & 1 else 0))) & U32
+        J[x] = (h2 << 32) | lo
+    M = jc ^ (eta if (s56 >> 30) & 1 else 0)
+    W = (W + wadd) & U32
+    if c:
+        s65 |= 0x80000000
+        W = (W + s56) & U32
+    for x in bits(M):
+        J[x] ^= (s65 << 32) | s64
+    return W
+
+
+def pack_end(s56, W):
+    """The packed term at the loop's exit (0 for a vanishing sum)."""
+    if s56 != END:
+        return 0
+    return ((W & 0x0FFFFFFF) + ((W >> 29) << 1)) & U32
+
+
+def packed_rows(r):
+    J = [(r.J[x] & MASK61) | ((r.d[x] & 7) << 61) for x in range(64)]
+    J[END] = 4 << 61
+    return J
+
+
+def batch_consts(r, n):
+    """The wave-uniform constants phiL_batch takes for a record with at most
+    two non-pivot columns."""
+    nonact = ((1 << n) - 1) & ~r.act
+    qs = list(bits(nonact))
+    col = [sum(1 << x for x in range(64) if (r.G[x] >> q) & 1) & r.act for q in qs] + [0, 0]
+    return dict(J0=packed_rows(r), actR=r.act | (1 << END) | (1 << TOP), actH=r.act & r.h,
+                D1=sum(1 << x for x in bits(r.act) if r.d[x] & 2),
+                D2=sum(1 << x for x in bits(r.act) if r.d[x] & 4),
+                nonact=nonact, col0=col[0], col1=col[1],
+                hq=sum(((r.h >> q) & 1) << b for b, q in enumerate(qs)), Q=(r.Q + 6) & 7, K16=16 * r.k + 1)
+
+
+def batch_ip(C, xj, count):
+    """One term of phiL_batch (cse_kernels.hip), register for register:
+    fixation, up to two restrictions, the loop, packing.  x_j carries bits
+    60 and 63 (so act = actR & x_j holds them, and the two extra counts in
+    n0 and |x_j| cancel or sit in Q)."""
+    xj |= (1 << END) | (1 << TOP)
+    J0 = C["J0"]
+    B = C["actH"] & ~xj
+    act = C["actR"] & xj
+    J, E = [0] * 64, 0
+    for x in range(64):
+        jb = J0[x] & B
+        E |= (popc(jb & ((1 << x) - 1)) & 1) << x
+        J[x] = (J0[x] + ((popc(jb) & 1) << 63)) & U64
+    n0 = popc(act)
+    m0 = C["Q"] + 2 * popc(C["D1"] & B) + n0 + 4 * popc((E ^ C["D2"]) & B)
+    W = (((m0 << 29) & U32) + (((popc(xj) - n0) << 4) + C["K16"])) & U32
+    Zn = C["nonact"] & ~xj
+    if Zn:
+        q0 = low(C["nonact"])
+        if (Zn >> q0) & 1:
+            col, h, Zn = C["col0"], C["hq"] & 1, Zn & ~(1 << q0)
+        else:
+            col, h, Zn = C["col1"], C["hq"] >> 1, 0
+        while True:
+            count["restrict"] += 1
+            S = col & act
+            beta = (popc(col & B) ^ h) & 1
+            if S:
+                i = low(S)
+                act &= ~(1 << i)
+                g = (C["col1"] >> i) & 1
+                ncol, nh = C["col1"] ^ (S if g else 0), (C["hq"] >> 1) ^ (beta if g else 0)
+                W = constrained_step(J, S, beta, W, 0xE0000010)
+            else:
+                if beta:
+                    return 0
+                ncol, nh = C["col1"], C["hq"] >> 1
+            if not Zn:
+                break
+            count["restrict2"] += 1
+            col, h, Zn = ncol, nh, 0
+    return pack_end(*elim_loop(J, act, W, count))
+
+
+def fallback_ip(r, xj, n, count):
+    """ip_phiL<true> with as_expsum_packed: the path of records with three or
+    more non-pivot columns (and the layout the exact-mode terms use)."""
     F = r.act & ~xj
     B = F & r.h
     s = r.copy()
@@ -181,7 +264,7 @@ def ip_phiL(r, xj, n, count):
         beta = ((s.h >> q) ^ popc(col & B)) & 1
         if not S:
             if beta:
-                return 0, 0, 0
+                return 0
             continue
         i = low(S)
         M, di = basis_change(s, i, S & ~(1 << i), True)
@@ -191,10 +274,20 @@ def ip_phiL(r, xj, n, count):
                 s.d[x] ^= 4
             s.h ^= s.G[i]
         s.act &= ~(1 << i)
-    eps, pp, mm = expsum_loop(s, count)
-    if not eps:
-        return 0, 0, 0
-    return 1, pp - r.k - popc(xj), mm
+    n0 = popc(s.act)
+    W = ((((s.Q + n0) << 29) & U32) + 16 * (64 - n0) + 1) & U32
+    s56, W = elim_loop(packed_rows(s), s.act | (1 << END) | (1 << TOP), W, count)
+    if s56 != END:
+        return 0
+    p = 64 - ((W & 0x0FFFFFFF) >> 4)
+    return 1 + 2 * ((r.k + popc(xj) - p) * 8 + (W >> 29))
+
+
+def decode(v):
+    if v == 0:
+        return (0, 0, 0)
+    v -= 1
+    return (1, -(v >> 4), (v >> 1) & 7)
 
 
 def xj_of(j, L, k):
@@ -205,7 +298,7 @@ def xj_of(j, L, k):
     return v
 
 
-def run_model(t, r_gens, k, seed, nth, nj):
+def run_model(t, r_gens, k, seed, nth, nj, only_fallback=False):
     from circuitsimulator_amd.workloads import commuting_paulis, random_L
     rng = np.random.default_rng(seed)
     ph, xs, zs = commuting_paulis(rng, t, r_gens)
@@ -222,16 +315,25 @@ def run_model(t, r_gens, k, seed, nth, nj):
         if orc.project(th, Po) < 0:
             continue
         rec = to_record(th)
+        batch = popc(((1 << t) - 1) & ~rec.act) <= 2
+        if batch and only_fallback:
+            continue
+        count["batch" if batch else "fallback"] += 1
+        C = batch_consts(rec, t) if batch else None
         for j in range(nj):
-            got = ip_phiL(rec, xj_of(j, L, k), t, count)
+            xj = xj_of(j, L, k)
             e, p, m = orc.inner(th, phis[j])
             want = (e, p, m % 8) if e else (0, 0, 0)
-            assert got == want, (t, st - 1, j, got, want)
+            got = decode(fallback_ip(rec, xj, t, count))
+            assert got == want, ("fallback", t, st - 1, j, got, want)
+            if batch:
+                got = decode(batch_ip(C, xj, count))
+                assert got == want, ("batch", t, st - 1, j, got, want)
             checked += 1
     return count
 
 
-@pytest.mark.parametrize("t,r,k,seed", [(48, 24, 12, 5), (20, 6, 8, 6), (9, 3, 4, 7), (61, 20, 10, 8)])
+@pytest.mark.parametrize("t,r,k,seed", [(48, 24, 12, 5), (20, 6, 8, 6), (9, 3, 4, 7), (60, 20, 10, 8)])
 def test_record_path_matches_oracle(t, r, k, seed):
     count = run_model(t, r, k, seed, nth=3, nj=min(24, 1 << k))
     # every kind of step of the loop ran at least once over the cases
@@ -242,5 +344,7 @@ def test_every_branch_is_exercised():
     count = collections.Counter()
     for t, r, k, seed in [(48, 24, 12, 11), (12, 3, 6, 12), (30, 4, 10, 13)]:
         count += run_model(t, r, k, seed, nth=3, nj=24)
-    for kind in ("A", "B", "C", "vanish", "restrict"):
+    # records with three or more non-pivot columns take the fallback path
+    count += run_model(12, 6, 6, 14, nth=2, nj=24, only_fallback=True)
+    for kind in ("A", "B", "C", "vanish", "restrict", "restrict2", "batch", "fallback"):
         assert count[kind] > 0, (kind, dict(count))
