@@ -378,9 +378,16 @@ DEV int ip_phiL(const RecView &rv, u64 lowx, u64 xj, u64 nonact) {
     return pack_term(eps, rv.k + __popcll(xj) - pp, mm);
 }
 
+DEV u64 lrow_combo(const uint64_t *L, int k, u64 j) {
+    u64 xv = 0;
+    for (int r = 0; r < k; r++)
+        if ((j >> (k - 1 - r)) & 1ull) xv ^= L[r];
+    return xv;
+}
+
 // Wave-uniform constants of one |L> record for phiL_batch: the pivot set
 // with bits 60 (the loop's end row) and 63 (kept in act by the loop), its
-// part fixed to 1 when outside supp(x_j) (act & h), the D bit planes, the
+// part fixed to 1 when outside supp(x_j) (act & h), the D planes, the
 // non-pivot columns (at most two), their column ballots over the pivot rows
 // (col0 for the lowest, col1 for the other) and h bits (hq bit 0, bit 1),
 // Q + 6 (the two extra bits of act count in n0), and 16 k + 1 with k the
@@ -390,14 +397,35 @@ struct PhiLConst {
     unsigned hq, Q, K16;
 };
 
-// nb terms <phi_j | theta'> for j = jnext - 1 .. jnext + nb - 2, packed, term
-// c in lane c of the result; x_j (which carries bits 60 and 63) advances by
-// flip[ctz(j + 1)] after every term (lane b of flip holds the XOR of rows
-// k-1 .. k-1-b of L).  One asm block computes what ip_phiL<true> computes,
-// per term:
+// Per-lane tables of a batch: lane c holds x_j of j = jb + c, given as xv
+// (with bits 60 and 63 set), and the term-independent part of W:
+// m = Q + n0 + 2 |D1 & B| in bits 29..31 and 16 (k + |x_j| - n0) + 1 below,
+// B = act & h & ~x_j, n0 = |act & x_j| (both extra bits count in n0 and in
+// |x_j|).
+DEV void phiL_tables(const PhiLConst &C, u64 xv, unsigned &xtlo, unsigned &xthi, unsigned &ttab) {
+    xv |= bit(CSE_END_ROW) | bit(63);
+    const unsigned n0 = (unsigned)__popcll(C.actR & xv);
+    const unsigned px = (unsigned)__popcll(xv);
+    const unsigned c1 = (unsigned)__popcll(C.D1 & C.actH & ~xv);
+    xtlo = (unsigned)xv;
+    xthi = (unsigned)(xv >> 32);
+    ttab = ((C.Q + n0 + 2u * c1) << 29) + 16u * (px - n0) + C.K16;
+}
+
+// The packed term of a parked W word (0 stays 0).
+DEV int phiL_unpack(int w) {
+    const unsigned u = (unsigned)w;
+    return u ? (int)((u & 0x0fffffffu) + ((u >> 29) << 1)) : 0;
+}
+
+// nb terms <phi_j | theta'>, j = jb .. jb + nb - 1, as raw W words (see
+// phiL_unpack), term c in lane c of the result; lane c of the tables is
+// phiL_tables' for j = jb + c.  One asm block computes what ip_phiL<true>
+// computes, per term:
 //  * fixation of the pivot variables outside supp(x_j): B = act & h & ~x_j;
 //    Q += sum_B D + 4 #edges(J inside B) (edges from the parity of J_x & B
-//    below x, one ballot), D_x += 4 |J_x & B| (bit 63 of row x);
+//    below x, one ballot; the D part is in the table), D_x += 4 |J_x & B|
+//    (bit 63 of row x);
 //  * the non-pivot coordinates outside supp(x_j), x_q = 0: with
 //    S = col_q & act the constraint S . z = h_q + |col_q & B| mod 2 is an
 //    empty sum (S = 0, right side 1), nothing (S = 0, right side 0), or an
@@ -405,28 +433,31 @@ struct PhiLConst {
 //    nothing (W += -1 in m, +1 in the exponent field); a second column then
 //    sees G_b ^= G_i on rows b in S, i.e. col1 ^= S and h_q1 ^= c when
 //    G_iq1 = 1;
-//  * the elimination loop, W starting at m = Q + n0 and 16 (k + |x_j| - n0)
-//    + 1, so that W's low field ends as the packed exponent;
-//  * packing: term = (W & 0x0fffffff) + 2 (W >> 29), 0 for a vanishing sum.
-// Fixed registers besides the loop's: m0 the term index (the lane
-// v_writelane writes; s47 keeps the caller's m0), s46 j + 1, s[44:45] B,
-// s[48:49] and s50 the column and h bit of the restriction at hand, s51 the
-// constraint's right side, s[52:53] the edge ballot, s54 a temporary,
-// s[62:63] the pending non-pivot columns before the loop.
-DEV int phiL_batch(u64 &xj, unsigned jnext, unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo,
-                   unsigned lxhi, unsigned flo, unsigned fhi, const PhiLConst &C) {
+//  * the elimination loop from W = table + 4 |(E ^ D2) & B| in m, whose low
+//    field ends as the packed exponent; 0 for a vanishing sum.
+// Fixed registers besides the loop's: m0 the term index (the lane the
+// tables are read from and v_writelane writes; s61 keeps the caller's m0),
+// s[46:47] x_j, s[44:45] B, s[48:49] and s50 the column and h bit of the
+// restriction at hand, s51 the constraint's right side / a temporary,
+// s[52:53] the edge ballot, s54 the table's W, s[62:63] the pending
+// non-pivot columns before the loop.
+DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, unsigned lxhi, unsigned xtlo,
+                   unsigned xthi, unsigned ttab, const PhiLConst &C) {
     int parked;
     unsigned jlo, jhi, vt0, vt1, vt2, vt3;
     asm volatile(
         "v_mov_b32 %[parked], 0\n\t"
         "v_mov_b32 v40, 0xc0000000\n\t"
-        "s_mov_b32 s47, m0\n\t"
+        "s_mov_b32 s61, m0\n\t"
         "s_mov_b32 m0, 0\n\t"
-        "s_mov_b32 s46, %[jnext]\n"
+        "s_nop 0\n"
         "L_ip%=:\n\t"
+        "v_readlane_b32 s46, %[xtlo], m0\n\t"
+        "v_readlane_b32 s47, %[xthi], m0\n\t"
+        "v_readlane_b32 s54, %[ttab], m0\n\t"
         // fixation
-        "s_andn2_b64 s[44:45], %[actH], %[xj]\n\t"
-        "s_and_b64 s[58:59], %[actR], %[xj]\n\t"
+        "s_andn2_b64 s[44:45], %[actH], s[46:47]\n\t"
+        "s_and_b64 s[58:59], %[actR], s[46:47]\n\t"
         "v_and_b32 %[vt0], s44, %[jlo0]\n\t"
         "v_and_b32 %[vt1], s45, %[jhi0]\n\t"
         "v_and_b32 %[vt3], %[vt1], %[lxhi]\n\t"
@@ -439,44 +470,27 @@ DEV int phiL_batch(u64 &xj, unsigned jnext, unsigned nb, unsigned jlo0, unsigned
         "v_cmp_ne_u32_e64 s[52:53], 0, %[vt2]\n\t"
         "v_mov_b32 %[jlo], %[jlo0]\n\t"
         "v_lshl_add_u32 %[jhi], %[vt3], 31, %[jhi0]\n\t"
-        "s_and_b64 s[62:63], %[D1], s[44:45]\n\t"
-        "s_bcnt1_i32_b64 s51, s[62:63]\n\t"
-        "s_bcnt1_i32_b64 s50, s[58:59]\n\t"
-        "s_lshl1_add_u32 s51, s51, %[Q]\n\t"
-        "s_add_u32 s51, s51, s50\n\t"
         "s_xor_b64 s[52:53], s[52:53], %[D2]\n\t"
         "s_and_b64 s[52:53], s[52:53], s[44:45]\n\t"
-        "s_bcnt1_i32_b64 s54, s[52:53]\n\t"
-        "s_lshl2_add_u32 s51, s54, s51\n\t"
-        "s_bcnt1_i32_b64 s54, %[xj]\n\t"
-        "s_sub_u32 s54, s54, s50\n\t"
-        "s_lshl4_add_u32 s54, s54, %[K16]\n\t"
-        "s_lshl_b32 s51, s51, 29\n\t"
-        "s_add_u32 s60, s51, s54\n\t"
-        "s_andn2_b64 s[62:63], %[nonact], %[xj]\n\t"
+        "s_bcnt1_i32_b64 s51, s[52:53]\n\t"
+        "s_lshl_b32 s51, s51, 31\n\t"
+        "s_add_u32 s60, s54, s51\n\t"
+        "s_andn2_b64 s[62:63], %[nonact], s[46:47]\n\t"
         "s_cbranch_scc1 L_rs%=\n"
         "L_elim%=:\n"
         CSE_ELIM_LOOP
         "L_end%=:\n\t"
         "s_cmp_eq_u32 s56, 60\n\t"
-        "s_cbranch_scc0 L_zero%=\n\t"
-        "s_lshr_b32 s51, s60, 29\n\t"
-        "s_and_b32 s60, s60, 0x0fffffff\n\t"
-        "s_lshl1_add_u32 s60, s51, s60\n\t"
-        "s_branch L_emit%=\n"
-        "L_zero%=:\n\t"
-        "s_mov_b32 s60, 0\n"
+        "s_cselect_b32 s60, s60, 0\n"
         "L_emit%=:\n\t"
         "v_writelane_b32 %[parked], s60, m0\n\t"
-        "s_ff1_i32_b32 s51, s46\n\t"
-        "s_add_u32 s46, s46, 1\n\t"
-        "v_readlane_b32 s62, %[flo], s51\n\t"
-        "v_readlane_b32 s63, %[fhi], s51\n\t"
         "s_add_u32 m0, m0, 1\n\t"
-        "s_xor_b64 %[xj], %[xj], s[62:63]\n\t"
         "s_cmp_lt_u32 m0, %[nb]\n\t"
         "s_cbranch_scc1 L_ip%=\n\t"
         "s_branch L_done%=\n"
+        "L_zero%=:\n\t"
+        "s_mov_b32 s60, 0\n\t"
+        "s_branch L_emit%=\n"
         // restrictions: s[62:63] = the non-pivot columns outside supp(x_j)
         "L_rs%=:\n\t"
         "s_ff1_i32_b64 s51, %[nonact]\n\t"
@@ -543,16 +557,15 @@ DEV int phiL_batch(u64 &xj, unsigned jnext, unsigned nb, unsigned jlo0, unsigned
         "s_lshr_b32 s50, %[hq], 1\n\t"
         "s_branch L_rnext%=\n"
         "L_done%=:\n\t"
-        "s_mov_b32 m0, s47\n\t"
+        "s_mov_b32 m0, s61\n\t"
         "s_nop 1\n\t"
-        : [parked] "=&v"(parked), [xj] "+s"(xj), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0),
-          [vt1] "=&v"(vt1), [vt2] "=&v"(vt2), [vt3] "=&v"(vt3)
-        : [jnext] "s"(jnext), [nb] "s"(nb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [lxlo] "v"(lxlo),
-          [lxhi] "v"(lxhi), [flo] "v"(flo), [fhi] "v"(fhi), [actR] "s"(C.actR), [actH] "s"(C.actH),
-          [D1] "s"(C.D1), [D2] "s"(C.D2), [nonact] "s"(C.nonact), [col0] "s"(C.col0), [col1] "s"(C.col1),
-          [hq] "s"(C.hq), [Q] "s"(C.Q), [K16] "s"(C.K16)
+        : [parked] "=&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
+          [vt2] "=&v"(vt2), [vt3] "=&v"(vt3)
+        : [nb] "s"(nb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [lxlo] "v"(lxlo), [lxhi] "v"(lxhi),
+          [xtlo] "v"(xtlo), [xthi] "v"(xthi), [ttab] "v"(ttab), [actR] "s"(C.actR), [actH] "s"(C.actH),
+          [D2] "s"(C.D2), [nonact] "s"(C.nonact), [col0] "s"(C.col0), [col1] "s"(C.col1), [hq] "s"(C.hq)
         : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
-          "s60", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+          "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
     return parked;
 }
 
@@ -588,12 +601,6 @@ DEV int ip_phiH(const RecView &rv, u64 j, int t) {
     return pack_term(eps, rv.k + k2 - pp, mm);
 }
 
-DEV u64 lrow_combo(const uint64_t *L, int k, u64 j) {
-    u64 xv = 0;
-    for (int r = 0; r < k; r++)
-        if ((j >> (k - 1 - r)) & 1ull) xv ^= L[r];
-    return xv;
-}
 
 // item -> instance by binary search on the item prefix
 DEV int find_instance(const long long *item_off, int ninst, long long item) {
@@ -780,16 +787,8 @@ DEV void sweep_body(const SweepArgs &A) {
                 flush(parked, nb, jb);
             }
         } else {
-            // x_j of |L> incrementally: going from j-1 to j flips bits 0..c
-            // of j, c = ctz(j), i.e. rows k-1 .. k-1-c of L; lane c holds
-            // their XOR
             const int k = D.k;
             const u64 nonact = lowmask(n) & ~rv.act;
-            u64 xj = lrow_combo(D.L, k, (u64)j0);
-            u64 flip = 0;
-            if (x < k) {
-                for (int b = 0; b <= x; b++) flip ^= D.L[k - 1 - b];
-            }
             if (PK && __popcll(nonact) <= 2) {
                 // the whole batch in one asm block
                 PhiLConst C;
@@ -806,16 +805,49 @@ DEV void sweep_body(const SweepArgs &A) {
                 C.Q = (unsigned)((rv.Q + 6) & 7);
                 C.K16 = 16u * (unsigned)rv.k + 1u;
                 const unsigned jlo0 = (unsigned)rv.J, jhi0 = (unsigned)(rv.J >> 32);
-                xj |= bit(CSE_END_ROW) | bit(63);
+                // x_{jb + c} for batches at multiples of 64: the part of the
+                // low six bits of j (lane c, xlow) and that of the others
+                // (uniform, xhigh, advanced from batch to batch by the rows
+                // whose bits flip: lane b of fliph holds the XOR of rows
+                // k-7 .. k-7-b)
+                const bool aligned = (j0 & 63) == 0;
+                u64 xlow = 0, fliph = 0, xhigh = 0;
+                if (aligned) {
+                    for (int b = 0; b < 6 && b < k; b++)
+                        if ((x >> b) & 1) xlow ^= D.L[k - 1 - b];
+                    if (x < k - 6)
+                        for (int b = 0; b <= x; b++) fliph ^= D.L[k - 7 - b];
+                    xhigh = lrow_combo(D.L, k, (u64)j0);
+                }
                 for (long long jb = j0; jb < j1; jb += 64) {
                     const int nb = (int)(j1 - jb < 64 ? j1 - jb : 64);
-                    // 0 < j + 1 <= chi <= 2^30
-                    const int parked = phiL_batch(xj, (unsigned)(jb + 1), (unsigned)nb, jlo0, jhi0,
-                                                  (unsigned)lowx, (unsigned)(lowx >> 32), (unsigned)flip,
-                                                  (unsigned)(flip >> 32), C);
+                    u64 xv;
+                    if (aligned) {
+                        xv = xlow ^ xhigh;
+                        // 0 < (jb >> 6) + 1 <= chi / 64 <= 2^24
+                        xhigh ^= rl64(fliph, __builtin_ctz((unsigned)((jb >> 6) + 1)));
+                    } else {
+                        // x_j = sum of the rows r of L with bit k-1-r of j
+                        // set, per lane
+                        const u64 j = (u64)(jb + x);
+                        xv = 0;
+                        for (int rr = 0; rr < k; rr++) xv ^= D.L[rr] & (0ull - ((j >> (k - 1 - rr)) & 1ull));
+                    }
+                    unsigned xtlo, xthi, ttab;
+                    phiL_tables(C, xv, xtlo, xthi, ttab);
+                    const int parked = phiL_unpack(phiL_batch((unsigned)nb, jlo0, jhi0, (unsigned)lowx,
+                                                              (unsigned)(lowx >> 32), xtlo, xthi, ttab, C));
                     flush(parked, nb, jb);
                 }
             } else {
+                // x_j of |L> incrementally: going from j-1 to j flips bits
+                // 0..c of j, c = ctz(j), i.e. rows k-1 .. k-1-c of L; lane c
+                // holds their XOR
+                u64 xj = lrow_combo(D.L, k, (u64)j0);
+                u64 flip = 0;
+                if (x < k) {
+                    for (int b = 0; b <= x; b++) flip ^= D.L[k - 1 - b];
+                }
                 for (long long jb = j0; jb < j1; jb += 64) {
                     const int nb = (int)(j1 - jb < 64 ? j1 - jb : 64);
                     int parked = 0;

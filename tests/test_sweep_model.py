@@ -10,10 +10,11 @@ Gaussian sum is eliminated one variable at a time by a hand-scheduled loop
 that ends at row 60 (J = 0, D = 4).
 
 This file restates that path in Python register for register: phiL_batch
-(the asm block of records with at most two non-pivot columns: fixation with
-one edge ballot, restrictions as constrained steps with the second column
-updated in scalar registers, the loop, packing from the W word m * 2^29 |
-16 np + 1), and the fallback of records with
+(the asm block of records with at most two non-pivot columns: per-lane
+tables of x_j and the term-independent part of the W word m * 2^29 |
+16 np + 1, fixation with one edge ballot, restrictions as constrained steps
+with the second column updated in scalar registers, the loop, W parked and
+unpacked later), and the fallback of records with
 more non-pivot columns (restrictions in C++, then the same loop).  Every term
 is checked against the oracle's innerProduct (libcirc/stabilizer/
 stabilizer.c:589-667), and the step kinds are counted so that every branch
@@ -173,11 +174,9 @@ def constrained_step(J, eta, c, W, wadd):
     return W
 
 
-def pack_end(s56, W):
-    """The packed term at the loop's exit (0 for a vanishing sum)."""
-    if s56 != END:
-        return 0
-    return ((W & 0x0FFFFFFF) + ((W >> 29) << 1)) & U32
+def unpack(W):
+    """phiL_unpack: the packed term of a parked W word (0 stays 0)."""
+    return ((W & 0x0FFFFFFF) + ((W >> 29) << 1)) & U32 if W else 0
 
 
 def packed_rows(r):
@@ -199,12 +198,21 @@ def batch_consts(r, n):
                 hq=sum(((r.h >> q) & 1) << b for b, q in enumerate(qs)), Q=(r.Q + 6) & 7, K16=16 * r.k + 1)
 
 
+def table_entry(C, xj):
+    """phiL_tables for one lane: x_j with bits 60 and 63 set (so act =
+    actR & x_j holds them, and the two extra counts in n0 and |x_j| cancel
+    or sit in Q) and the term-independent part of W."""
+    xv = xj | (1 << END) | (1 << TOP)
+    n0 = popc(C["actR"] & xv)
+    c1 = popc(C["D1"] & C["actH"] & ~xv)
+    T = (((C["Q"] + n0 + 2 * c1) << 29) + 16 * (popc(xv) - n0) + C["K16"]) & U32
+    return xv, T
+
+
 def batch_ip(C, xj, count):
     """One term of phiL_batch (cse_kernels.hip), register for register:
-    fixation, up to two restrictions, the loop, packing.  x_j carries bits
-    60 and 63 (so act = actR & x_j holds them, and the two extra counts in
-    n0 and |x_j| cancel or sit in Q)."""
-    xj |= (1 << END) | (1 << TOP)
+    fixation, up to two restrictions, the loop; returns the parked W word."""
+    xj, T = table_entry(C, xj)
     J0 = C["J0"]
     B = C["actH"] & ~xj
     act = C["actR"] & xj
@@ -213,9 +221,7 @@ def batch_ip(C, xj, count):
         jb = J0[x] & B
         E |= (popc(jb & ((1 << x) - 1)) & 1) << x
         J[x] = (J0[x] + ((popc(jb) & 1) << 63)) & U64
-    n0 = popc(act)
-    m0 = C["Q"] + 2 * popc(C["D1"] & B) + n0 + 4 * popc((E ^ C["D2"]) & B)
-    W = (((m0 << 29) & U32) + (((popc(xj) - n0) << 4) + C["K16"])) & U32
+    W = (T + (popc((E ^ C["D2"]) & B) << 31)) & U32
     Zn = C["nonact"] & ~xj
     if Zn:
         q0 = low(C["nonact"])
@@ -241,7 +247,8 @@ def batch_ip(C, xj, count):
                 break
             count["restrict2"] += 1
             col, h, Zn = ncol, nh, 0
-    return pack_end(*elim_loop(J, act, W, count))
+    s56, W = elim_loop(J, act, W, count)
+    return W if s56 == END else 0
 
 
 def fallback_ip(r, xj, n, count):
@@ -327,7 +334,7 @@ def run_model(t, r_gens, k, seed, nth, nj, only_fallback=False):
             got = decode(fallback_ip(rec, xj, t, count))
             assert got == want, ("fallback", t, st - 1, j, got, want)
             if batch:
-                got = decode(batch_ip(C, xj, count))
+                got = decode(unpack(batch_ip(C, xj, count)))
                 assert got == want, ("batch", t, st - 1, j, got, want)
             checked += 1
     return count
