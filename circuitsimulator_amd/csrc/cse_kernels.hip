@@ -390,8 +390,7 @@ DEV u64 lrow_combo(const uint64_t *L, int k, u64 j) {
 // part fixed to 1 when outside supp(x_j) (act & h), the D planes, the
 // non-pivot columns (at most two), their column ballots over the pivot rows
 // (col0 for the lowest, col1 for the other) and h bits (hq bit 0, bit 1),
-// Q + 6 (the two extra bits of act count in n0), and 16 k + 1 with k the
-// record's dimension.
+// Q, and 16 k + 1 with k the record's dimension.
 struct PhiLConst {
     u64 actR, actH, D1, D2, nonact, col0, col1;
     unsigned hq, Q, K16;
@@ -399,7 +398,7 @@ struct PhiLConst {
 
 // Per-lane tables of a batch: lane c holds x_j of j = jb + c, given as xv
 // (with bits 60 and 63 set), and the term-independent part of W:
-// m = Q + n0 + 2 |D1 & B| in bits 29..31 and 16 (k + |x_j| - n0) + 1 below,
+// m = Q + 2 |D1 & B| in bits 29..31 and 16 (k + |x_j| - n0) + 1 below,
 // B = act & h & ~x_j, n0 = |act & x_j| (both extra bits count in n0 and in
 // |x_j|).
 DEV void phiL_tables(const PhiLConst &C, u64 xv, unsigned &xtlo, unsigned &xthi, unsigned &ttab) {
@@ -409,7 +408,7 @@ DEV void phiL_tables(const PhiLConst &C, u64 xv, unsigned &xtlo, unsigned &xthi,
     const unsigned c1 = (unsigned)__popcll(C.D1 & C.actH & ~xv);
     xtlo = (unsigned)xv;
     xthi = (unsigned)(xv >> 32);
-    ttab = ((C.Q + n0 + 2u * c1) << 29) + 16u * (px - n0) + C.K16;
+    ttab = ((C.Q + 2u * c1) << 29) + 16u * (px - n0) + C.K16;
 }
 
 // The packed term of a parked W word (0 stays 0).
@@ -429,8 +428,8 @@ DEV int phiL_unpack(int w) {
 //  * the non-pivot coordinates outside supp(x_j), x_q = 0: with
 //    S = col_q & act the constraint S . z = h_q + |col_q & B| mod 2 is an
 //    empty sum (S = 0, right side 1), nothing (S = 0, right side 0), or an
-//    elimination of min(S) as a constrained step whose variable counts
-//    nothing (W += -1 in m, +1 in the exponent field); a second column then
+//    elimination of min(S) as a constrained step whose variable leaves the
+//    exponent (W += 16) and adds no factor; a second column then
 //    sees G_b ^= G_i on rows b in S, i.e. col1 ^= S and h_q1 ^= c when
 //    G_iq1 = 1;
 //  * the elimination loop from W = table + 4 |(E ^ D2) & B| in m, whose low
@@ -537,7 +536,7 @@ DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, uns
         "s_cbranch_scc0 L_rnojii%=\n\t"
         "v_xor_b32 %[vt3], %[vt3], %[vt0]\n"
         "L_rnojii%=:\n\t"
-        "s_add_u32 s60, s60, 0xe0000010\n\t"
+        "s_add_u32 s60, s60, 0x10\n\t"
         "s_bitcmp1_b32 s51, 0\n\t"
         "s_cbranch_scc0 L_rnoc%=\n\t"
         "s_or_b32 s65, s65, 0x80000000\n\t"
@@ -802,7 +801,7 @@ DEV void sweep_body(const SweepArgs &A) {
                 C.col0 = q0m ? (ballot((rv.G >> q0) & 1ull) & rv.act) : 0ull;
                 C.col1 = q1m ? (ballot((rv.G >> q1) & 1ull) & rv.act) : 0ull;
                 C.hq = (unsigned)((q0m ? (rv.h >> q0) & 1ull : 0ull) | (q1m ? ((rv.h >> q1) & 1ull) << 1 : 0ull));
-                C.Q = (unsigned)((rv.Q + 6) & 7);
+                C.Q = (unsigned)rv.Q;
                 C.K16 = 16u * (unsigned)rv.k + 1u;
                 const unsigned jlo0 = (unsigned)rv.J, jhi0 = (unsigned)(rv.J >> 32);
                 // x_{jb + c} for batches at multiples of 64: the part of the

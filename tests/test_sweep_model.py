@@ -130,8 +130,7 @@ def elim_loop(J, act, W, count):
                 lo = (J[x] & U32) ^ (eta & U32)
                 h2 = (((J[x] >> 32) ^ (eta >> 32)) + 0xC0000000) & U32
                 J[x] = (h2 << 32) | lo
-            if (hi >> 31) & 1:                   # D_a = 6: m -= 2
-                W = (W + 0xC0000000) & U32
+            W = (W + (0xE0000000 if (hi >> 31) & 1 else 0x20000000)) & U32
             continue
         if not (eta << 1) & U64:                 # no constraint
             if (hi >> 31) & 1:                   # D_a = 4, or the end row
@@ -139,33 +138,68 @@ def elim_loop(J, act, W, count):
                     count["vanish"] += 1
                 return a, W
             count["C"] += 1
-            W = (W + 0xDFFFFFF0) & U32
+            W = (W + 0xFFFFFFF0) & U32
             continue
         count["B"] += 1                          # constraint eta . z = D_a/4
-        W = constrained_step(J, eta, (hi >> 31) & 1, W, 0xC0000000)
         act &= ~(1 << low(eta))
+        i = low(eta)
+        Jr = J[i]
+        s66, s67 = Jr & U32, Jr >> 32
+        s56 = s67 & 0xE0000000
+        s64, s65 = eta & U32, eta >> 32          # s65 bit 31: c
+        c = (hi >> 31) & 1
+        sp = [(eta >> x) & 1 for x in range(64)]
+        if not (s67 >> 30) & 1:                  # D_i in {0,4}: bit 63 of J_i adds it
+            count["B0"] += 1
+            jc = (s67 << 32) | s66
+            for x in range(64):
+                if sp[x]:
+                    J[x] ^= jc
+            if c:
+                W = (W + s56) & U32
+            for x in range(64):
+                if (jc >> x) & 1:
+                    vt1 = s65 ^ (0x80000000 if sp[x] else 0)
+                    J[x] ^= (vt1 << 32) | s64
+        else:                                    # D_i in {2,6}
+            count["B1"] += 1
+            s67 &= 0x1FFFFFFF
+            jc = (s67 << 32) | s66
+            for x in range(64):
+                lo, h2 = J[x] & U32, J[x] >> 32
+                if sp[x]:
+                    lo ^= s66
+                    h2 ^= s67
+                    h2 = (h2 + (s56 ^ (0x80000000 if (jc >> x) & 1 else 0))) & U32
+                J[x] = (h2 << 32) | lo
+            if c:
+                W = (W + s56) & U32
+            M = jc ^ eta
+            for x in bits(M):
+                J[x] ^= (s65 << 32) | s64
 
 
-def constrained_step(J, eta, c, W, wadd):
-    """Elimination of i = min(eta) over Sp = eta with z_i = c, register for
-    register: rows in Sp take J ^= Jc and D += D_i + 4 J_xi, rows in M take
-    J ^= Sp (and D += 4 when c, through bit 63 of Sp), W += wadd (+ D_i)."""
-    i = low(eta)
+def restrict_step(J, S, c, W):
+    """The restriction's elimination of i = min(S) over Sp = S with z_i = c
+    (the asm's copy of the constrained step, S without bit 63): rows in Sp
+    take J ^= Jc and D += D_i + 4 J_xi, rows in M take J ^= Sp (D += 4 when
+    c), W += 16 (the variable leaves the exponent) + c D_i."""
+    i = low(S)
     Jr = J[i]
     s66, s67 = Jr & U32, Jr >> 32
     s56 = s67 & 0xE0000000
     s67 &= 0x1FFFFFFF
     jc = (s67 << 32) | s66
-    s64, s65 = eta & U32, eta >> 32
+    s64, s65 = S & U32, S >> 32
     for x in range(64):
         lo, h2 = J[x] & U32, J[x] >> 32
-        if (eta >> x) & 1:
+        if (S >> x) & 1:
             lo ^= s66
             h2 ^= s67
             h2 = (h2 + (s56 ^ (0x80000000 if (jc >> x) & 1 else 0))) & U32
         J[x] = (h2 << 32) | lo
-    M = jc ^ (eta if (s56 >> 30) & 1 else 0)
-    W = (W + wadd) & U32
+    M = jc ^ (S if (s56 >> 30) & 1 else 0)
+    W = (W + 0x10) & U32
     if c:
         s65 |= 0x80000000
         W = (W + s56) & U32
@@ -195,17 +229,17 @@ def batch_consts(r, n):
                 D1=sum(1 << x for x in bits(r.act) if r.d[x] & 2),
                 D2=sum(1 << x for x in bits(r.act) if r.d[x] & 4),
                 nonact=nonact, col0=col[0], col1=col[1],
-                hq=sum(((r.h >> q) & 1) << b for b, q in enumerate(qs)), Q=(r.Q + 6) & 7, K16=16 * r.k + 1)
+                hq=sum(((r.h >> q) & 1) << b for b, q in enumerate(qs)), Q=r.Q & 7, K16=16 * r.k + 1)
 
 
 def table_entry(C, xj):
     """phiL_tables for one lane: x_j with bits 60 and 63 set (so act =
-    actR & x_j holds them, and the two extra counts in n0 and |x_j| cancel
-    or sit in Q) and the term-independent part of W."""
+    actR & x_j holds them; their counts in n0 and |x_j| cancel) and the
+    term-independent part of W."""
     xv = xj | (1 << END) | (1 << TOP)
     n0 = popc(C["actR"] & xv)
     c1 = popc(C["D1"] & C["actH"] & ~xv)
-    T = (((C["Q"] + n0 + 2 * c1) << 29) + 16 * (popc(xv) - n0) + C["K16"]) & U32
+    T = (((C["Q"] + 2 * c1) << 29) + 16 * (popc(xv) - n0) + C["K16"]) & U32
     return xv, T
 
 
@@ -238,7 +272,7 @@ def batch_ip(C, xj, count):
                 act &= ~(1 << i)
                 g = (C["col1"] >> i) & 1
                 ncol, nh = C["col1"] ^ (S if g else 0), (C["hq"] >> 1) ^ (beta if g else 0)
-                W = constrained_step(J, S, beta, W, 0xE0000010)
+                W = restrict_step(J, S, beta, W)
             else:
                 if beta:
                     return 0
@@ -282,7 +316,7 @@ def fallback_ip(r, xj, n, count):
             s.h ^= s.G[i]
         s.act &= ~(1 << i)
     n0 = popc(s.act)
-    W = ((((s.Q + n0) << 29) & U32) + 16 * (64 - n0) + 1) & U32
+    W = (((s.Q << 29) & U32) + 16 * (64 - n0) + 1) & U32
     s56, W = elim_loop(packed_rows(s), s.act | (1 << END) | (1 << TOP), W, count)
     if s56 != END:
         return 0
@@ -353,5 +387,5 @@ def test_every_branch_is_exercised():
         count += run_model(t, r, k, seed, nth=3, nj=24)
     # records with three or more non-pivot columns take the fallback path
     count += run_model(12, 6, 6, 14, nth=2, nj=24, only_fallback=True)
-    for kind in ("A", "B", "C", "vanish", "restrict", "restrict2", "batch", "fallback"):
+    for kind in ("A", "B", "B0", "B1", "C", "vanish", "restrict", "restrict2", "batch", "fallback"):
         assert count[kind] > 0, (kind, dict(count))
