@@ -28,7 +28,7 @@ SALU_INSTS_PER_IP = 160.5    # SQ_INSTS_SALU / inner products
 VALU_LANE_OPS_PER_IP = VALU_INSTS_PER_IP * 64
 # HBM bytes per inner product of the sweep: (FETCH_SIZE x 2.000 + WRITE_SIZE x
 # 1.000) KiB, factors from a 1 GiB copy kernel, / inner products
-SWEEP_BYTES_PER_IP = 2.83
+SWEEP_BYTES_PER_IP = 2.76
 CLOCK_GHZ = 2.38             # GRBM_GUI_ACTIVE / kernel time / 8 XCDs
 PEAK_INT_TOPS = 39.3  # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz, int32 lane-ops/s
 
