@@ -537,7 +537,8 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
     "s_branch L_top%=\n"
 
 // The loop on an AState (records of n <= 60, row 60 the end row); p and m
-// as as_expsum.  The base of W's low field is 64, so p = 64 - low / 16.
+// as as_expsum.  W's low field counts down from the base 128 (p = n0 + nC
+// <= 2 n0 <= 120), so p = 128 - low / 16.
 DEV int as_expsum_packed(AState &s, int &p, int &m) {
     const u64 act0 = uniform64(s.act);
     u64 act = act0 | bit(CSE_END_ROW) | bit(63);
@@ -545,7 +546,7 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
     const int n0 = __popcll(act0);
     unsigned jlo = (unsigned)s.J, jhi = (unsigned)(s.J >> 32);
     unsigned W = (unsigned)__builtin_amdgcn_readfirstlane(
-        (int)(((unsigned)s.Q << 29) + 16u * (unsigned)(64 - n0) + 1u));
+        (int)(((unsigned)s.Q << 29) + 16u * (unsigned)(128 - n0) + 1u));
     unsigned vt0, vt1, vt2, vt3;
     int endv;
     asm volatile(
@@ -562,7 +563,7 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
         m = 0;
         return 0;
     }
-    p = 64 - (int)((W & 0x0fffffffu) >> 4);
+    p = 128 - (int)((W & 0x0fffffffu) >> 4);
     m = (int)(W >> 29);
     return 1;
 }

@@ -218,6 +218,30 @@ def test_batch_uneven_instances(E):
         assert res[q].im == 0.0
 
 
+def test_exact_diagonal_exponents_above_64(E):
+    # an uneven batch instance (t=48, r=14, k=7): on the diagonal terms a
+    # projected |L> state meets itself, and p = n0 + nC reaches ~88
+    from circuitsimulator_amd.workloads import uneven_instances
+    inst, specs = uneven_instances(140, seed=1003)
+    t, r, k, ph, xs, zs, L = specs[139]
+    chi = 1 << k
+    res, terms, _ = E.batch_exact([inst[139]], want_terms=True, nterms=chi * (chi + 1) // 2)
+    diag = [i * chi - i * (i - 1) // 2 for i in range(chi)]
+    want = oracle_exact_terms(t, k, [int(v) for v in L], 0, ph, xs, zs, ls=diag)
+    assert [int(terms[l]) for l in diag] == [w[0] for w in want]
+    assert res[0].im == 0.0
+
+
+def test_batch_uneven_large_t_runs_clean(E):
+    # the first 330 instances of the uneven batch workload (t up to 48, k up
+    # to 14): every total exactly real and non-negative, no term out of range
+    from circuitsimulator_amd.workloads import uneven_instances
+    inst, specs = uneven_instances(330, seed=1003)
+    res, _, _ = E.batch_exact(inst)
+    assert all(r.im == 0.0 and r.re >= 0.0 for r in res)
+    assert [r.terms for r in res] == [(1 << k) * ((1 << k) + 1) // 2 for (_, _, k, *_x) in specs]
+
+
 def test_sampled_t64_unpacked_layout(E):
     # n > 61 runs the sweep with D in its own register
     from circuitsimulator_amd.engine import Projector

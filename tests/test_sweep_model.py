@@ -316,11 +316,11 @@ def fallback_ip(r, xj, n, count):
             s.h ^= s.G[i]
         s.act &= ~(1 << i)
     n0 = popc(s.act)
-    W = (((s.Q << 29) & U32) + 16 * (64 - n0) + 1) & U32
+    W = (((s.Q << 29) & U32) + 16 * (128 - n0) + 1) & U32
     s56, W = elim_loop(packed_rows(s), s.act | (1 << END) | (1 << TOP), W, count)
     if s56 != END:
         return 0
-    p = 64 - ((W & 0x0FFFFFFF) >> 4)
+    p = 128 - ((W & 0x0FFFFFFF) >> 4)
     return 1 + 2 * ((r.k + popc(xj) - p) * 8 + (W >> 29))
 
 
@@ -389,3 +389,32 @@ def test_every_branch_is_exercised():
     count += run_model(12, 6, 6, 14, nth=2, nj=24, only_fallback=True)
     for kind in ("A", "B", "B0", "B1", "C", "vanish", "restrict", "restrict2", "batch", "fallback"):
         assert count[kind] > 0, (kind, dict(count))
+
+
+def test_exponent_above_64_on_diagonal_terms():
+    """A projected |L> state against itself: with t = 48 its record keeps
+    up to ~44 free variables that all leave through steps without a
+    constraint, so p = n0 + nC reaches ~88 (the exact-mode diagonal terms of
+    an uneven batch instance, t=48 r=14 k=7)."""
+    from circuitsimulator_amd.workloads import uneven_instances
+    _, specs = uneven_instances(140, seed=1003)
+    t, r, k, ph, xs, zs, L = specs[139]
+    assert (t, r, k) == (48, 14, 7)
+    Po = orc.make_projector(ph, xs, zs, t)
+    count = collections.Counter()
+    pmax = 0
+    for i in range(1 << k):
+        th = orc.prepL(i, t, L)
+        if orc.project(th, Po) < 0:
+            continue
+        rec = to_record(th)
+        e, p, m = orc.inner(th, orc.prepL(i, t, L))
+        want = (e, p, m % 8) if e else (0, 0, 0)
+        xj = xj_of(i, L, k)
+        assert decode(fallback_ip(rec, xj, t, count)) == want, i
+        if popc(((1 << t) - 1) & ~rec.act) <= 2:
+            assert decode(unpack(batch_ip(batch_consts(rec, t), xj, count))) == want, i
+        if e:
+            pmax = max(pmax, rec.k + popc(xj) + p)  # n0 + nC of the sweep
+    # the case that needs the exponent field's base above 64
+    assert pmax > 64
