@@ -278,6 +278,31 @@ def test_sampled_t48_workload_subsample(E):
     assert res.ips == int((ex["sexp"] >= 0).sum()) * chi
 
 
+def test_sampled_exact_decomposition_t44_subsample(E):
+    # |H^t> terms at t=44 (chi = 2^22): phi_0 has every pair coupled and no
+    # restriction, so with theta it can reach p = n0 + nC above 64
+    from circuitsimulator_amd.engine import Projector
+    t, r = 44, 10
+    rng = np.random.default_rng(4401)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    chi = 1 << ((t + 1) // 2)
+    count = 2
+    res, ex = E.sampled_projector(Projector.make(ph, xs, zs, t), [], 0, 1, 91, 3, count, want_terms=True)
+    Po = orc.make_projector(ph, xs, zs, t)
+    thr = orc.thresholds(t)
+    js = [0, 1, chi - 1] + [int(v) for v in rng.integers(0, chi, size=60)]
+    for s in range(count):
+        th = orc.random_state(t, 91, 3 + s, thr)
+        e = orc.project(th, Po)
+        assert ex["sexp"][s] == e
+        if e < 0:
+            continue
+        for j in js:
+            eps, p, m = orc.inner(th, orc.prepH(j, t))
+            assert ex["terms"][s][j] == pack_term(eps, p, m), (s, j)
+    assert res.ips == int((ex["sexp"] >= 0).sum()) * chi
+
+
 def test_exact_t24_workload_subsample(E):
     # exact mode at t=24, r=12, k=11: chi(chi+1)/2 = 2,098,176 pair terms
     from circuitsimulator_amd.engine import Instance
