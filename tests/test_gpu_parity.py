@@ -242,6 +242,27 @@ def test_batch_uneven_large_t_runs_clean(E):
     assert [r.terms for r in res] == [(1 << k) * ((1 << k) + 1) // 2 for (_, _, k, *_x) in specs]
 
 
+def test_uneven_workload_seeded_subsample(E):
+    # the uneven batch workload itself (4,096 instances, t up to 48): 64
+    # seeded instances with k <= 10 in one launch, and per instance a seeded
+    # subsample of its pair terms (always the first and last) against the
+    # oracle
+    from circuitsimulator_amd.workloads import uneven_instances
+    inst, specs = uneven_instances(4096, seed=1003)
+    rng = np.random.default_rng(3003)
+    pick = [int(q) for q in rng.permutation(len(inst)) if specs[q][2] <= 10][:64]
+    sizes = [(1 << specs[q][2]) * ((1 << specs[q][2]) + 1) // 2 for q in pick]
+    res, terms, _ = E.batch_exact([inst[q] for q in pick], want_terms=True, nterms=sum(sizes))
+    off = 0
+    for q, nl in zip(pick, sizes):
+        t, r, k, ph, xs, zs, L = specs[q]
+        ls = sorted(set([0, nl - 1] + [int(v) for v in rng.integers(0, nl, size=120)]))
+        want = oracle_exact_terms(t, k, [int(v) for v in L], 0, ph, xs, zs, ls=ls)
+        assert [int(terms[off + l]) for l in ls] == [w[0] for w in want], q
+        off += nl
+    assert all(r.im == 0.0 for r in res)
+
+
 def test_sampled_t64_unpacked_layout(E):
     # n > 61 runs the sweep with D in its own register
     from circuitsimulator_amd.engine import Projector
