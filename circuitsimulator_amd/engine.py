@@ -66,6 +66,16 @@ class Instance(ctypes.Structure):
     _fields_ = [("P", Projector), ("exact", ctypes.c_int), ("k", ctypes.c_int), ("L", U64 * NMAX)]
 
 
+# the same layout as a numpy record: arrays of instances are filled with
+# array operations and passed to cse_batch_exact by pointer
+INSTANCE_DTYPE = np.dtype(Instance)
+
+
+def instance_array(n):
+    """n zeroed struct cse_instance records as a numpy array."""
+    return np.zeros(n, dtype=INSTANCE_DTYPE)
+
+
 _lib = None
 
 
@@ -215,7 +225,13 @@ class Engine:
         return hist
 
     def batch_exact(self, instances, chunk=0, want_terms=False, nterms=0):
-        arr = (Instance * len(instances))(*instances)
+        """instances: a list of Instance or an instance_array."""
+        if isinstance(instances, np.ndarray):
+            if instances.dtype != INSTANCE_DTYPE or not instances.flags["C_CONTIGUOUS"]:
+                raise ValueError("instances must be a contiguous instance_array")
+            arr = instances.ctypes.data_as(P(Instance))
+        else:
+            arr = (Instance * len(instances))(*instances)
         res = (ExactResult * len(instances))()
         terms = np.zeros(nterms, dtype=np.int32) if want_terms else None
         ms = ctypes.c_double()

@@ -84,6 +84,33 @@ def decompose(t, config, seed=0):
     return rows, norm.value, bool(exact.value), k.value
 
 
+def decompose_many(t, config, seeds):
+    """decompose once per seed with the same config: (L rows as a uint64
+    array of shape (len(seeds), k), or None for the exact decomposition,
+    norm, exact, k).  exact, k and norm depend on t and config only."""
+    import ctypes
+    from .engine import NMAX, U64
+    L = (U64 * NMAX)()
+    norm = ctypes.c_double()
+    args = (float(config.get("fidbound") or 1e-5), 1 if config.get("rank") else 0,
+            1 if config.get("fidelity") else 0, 1 if config.get("forceL") else 0,
+            1 if config.get("verbose") else 0, 1 if config.get("quiet") else 0)
+    f = lib().cse_decompose
+    out, ex0, k0 = None, None, None
+    for n, seed in enumerate(seeds):
+        exact = ctypes.c_int(1 if config.get("exact") else 0)
+        k = ctypes.c_int(int(config.get("k") or 0))
+        if f(int(t), int(seed) & _SEED_MASK, L, ctypes.byref(norm), ctypes.byref(exact), ctypes.byref(k), *args):
+            raise ValueError("decomposition of t=%d is outside the engine's range" % t)
+        if out is None:
+            ex0, k0 = bool(exact.value), k.value
+            if ex0:
+                return None, norm.value, True, k0
+            out = np.zeros((len(seeds), k0), dtype=np.uint64)
+        out[n] = np.ctypeslib.as_array(L)[:k0]
+    return out, norm.value, ex0, k0
+
+
 def _same(Gp, Hp):
     # probability.py:140-150
     same = True

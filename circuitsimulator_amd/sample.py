@@ -128,6 +128,23 @@ def _shot_projector(prods, values):
     return keep_ph, keep_x, keep_z
 
 
+def _round_phases(prods, V):
+    """Every shot's G' at once: _shot_projector for the outcome rows of V (one
+    per shot, padded with 0 for the T-gadget rows).  The products' x and z
+    masks are shared, so a shot's G' differs only in its phases.  Returns the
+    phases of the kept generators (shots x g), their x and z masks, and which
+    shots' G' contain -identity (the outcome is impossible)."""
+    ph0, xm, zm, X, _ = prods
+    V = np.asarray(V, dtype=np.int64)
+    PH = (np.asarray(ph0, dtype=np.int64)[None, :] + 2 * (V @ X.T.astype(np.int64))) % 4
+    ident = np.array([xm[j] == 0 and zm[j] == 0 for j in range(len(xm))], dtype=bool)
+    if (PH[:, ident] % 2).any():
+        raise AssertionError("a product of commuting Hermitian generators has phase +-i on the identity")
+    impossible = (PH[:, ident] == 2).any(axis=1)
+    keep = np.nonzero(~ident)[0]
+    return PH[:, keep], [xm[j] for j in keep], [zm[j] for j in keep], impossible
+
+
 def _bits(P, t):
     """0/1 arrays of a mask-form projector (for probability()'s helpers)."""
     ph, xm, zm = P
@@ -145,13 +162,15 @@ def sampleShots(circ, measure, sample, shots, config={}):
     outcomes is a subset of the rows of one gadgetization, with the phase
     +2 on outcome 1.  Per round the null space of the truncation and the
     generator masks are shared by all shots (they depend on the X and Z
-    parts only) and a shot's G' differs in its phases.  Each call's
-    decomposition is drawn afresh, as probability() does, and all numerators
+    parts only) and a shot's G' differs in its phases, which one matrix
+    product gives for all shots.  Each call's decomposition is drawn
+    afresh, as probability() does (one seed per call), and all numerators
     and denominators of a round are evaluated by one batched engine launch
-    (cse_batch_exact) whenever the reference's rule evaluates them exactly
-    (probability.c:162-174; with the default 2000 samples this is every |L>
-    of k <= 11 and every exact |H> of t <= 22); otherwise each norm is
-    sampled on its own.
+    (cse_batch_exact, its instances written as one array) whenever the
+    reference's rule evaluates them exactly (probability.c:162-174; with the
+    default 2000 samples this is every |L> of k <= 11 and every exact |H> of
+    t <= 22); with the exact decomposition every distinct G' and H' is
+    evaluated once.  Otherwise each norm is sampled on its own.
 
     Differences from calling sampleQubits shots times, none of which changes
     a probability: one T-ancilla postselection y serves the whole run (any y
@@ -163,8 +182,8 @@ def sampleShots(circ, measure, sample, shots, config={}):
     when the constraints cannot be met).
     """
     from .compile.compilecirc import standardGate
-    from .engine import NMAX, Engine, Instance, Projector, sampling_kept
-    from .probability import _pick_sampling, projector_norms
+    from .engine import Engine, instance_array, sampling_kept
+    from .probability import _pick_sampling, decompose_many, projector_norms
 
     t, Ms, _ = _gadgetize.countY(circ)
     rng = config.get("rng")
@@ -195,93 +214,116 @@ def sampleShots(circ, measure, sample, shots, config={}):
     device = int(cfg.get("device") or 0)
     engine = None
 
-    def same_as_H(P):
-        # probability.py:140-150: the first len(G') generators of H' agree
-        return len(P[0]) <= len(Hm[0]) and all(
-            P[0][i] == Hm[0][i] and P[1][i] == Hm[1][i] and P[2][i] == Hm[2][i] for i in range(len(P[0])))
+    def fill(A, rows, phases, xs, zs, ex, k, Ls):
+        """Records rows of A: projector (phases: one row per record or one
+        shared row), decomposition (Ls: one row of L per record, or None)."""
+        g = len(xs)
+        A["P"]["nstabs"][rows] = g
+        A["P"]["nqubits"][rows] = t
+        if g:
+            A["P"]["phase"][rows, :g] = np.asarray(phases, dtype=np.int64) & 3
+            A["P"]["xs"][rows, :g] = np.array(xs, dtype=np.uint64)
+            A["P"]["zs"][rows, :g] = np.array(zs, dtype=np.uint64)
+        A["exact"][rows] = 1 if ex else 0
+        A["k"][rows] = k
+        if Ls is not None and k:
+            A["L"][rows, :k] = Ls
 
-    def evaluate(calls, uu, exact):
-        """P of each call; calls hold one G' (mask form) or None each."""
+    def evaluate(PH, xk, zk, impossible, uu, exact):
+        """P of every shot of a round: PH, xk, zk, impossible as
+        _round_phases returns them."""
         nonlocal engine
-        out = [0.0] * len(calls)
-        inst, pending = [], []
-        for c, P in enumerate(calls):
-            if P is None:
-                continue  # -identity: P = 0
-            if same_as_H(P):
-                out[c] = 2.0 ** (v - uu)
-                continue
-            callcfg = dict(cfg)
-            # an empty projector needs the exact decomposition's norm
-            callcfg["exact"] = bool(exact) or len(P[0]) == 0 or len(Hm[0]) == 0
-            seed = int(rand_int(2 ** 62, 1)[0])
-            L, norm, ex, k = decompose(t, callcfg, seed)
-            if t == 0 or sampling_kept(callcfg.get("noapprox"), callcfg.get("forceSample"), ex, t, k,
-                                       int(callcfg.get("samples") or 0), int(callcfg.get("bins") or 0)):
-                # sampled norms, or the t = 0 closed form: one call each
-                callcfg["seed"] = seed
-                num, den = projector_norms(_bits(P, t), _bits(Hm, t), t, callcfg)
-                out[c] = 0.0 if num == 0 else 2.0 ** (v - uu) * num / den
-                continue
-            slots = []
-            for Q in (P, Hm):
-                if len(Q[0]) == 0:
-                    slots.append(None)
-                    continue
-                ins = Instance()
-                ins.P = Projector.make(Q[0], Q[1], Q[2], t)
-                ins.exact = 1 if ex else 0
-                ins.k = k
-                for i, r in enumerate(L[:NMAX]):
-                    ins.L[i] = r
-                slots.append(len(inst))
-                inst.append(ins)
-            pending.append((c, norm, slots))
-        if inst:
-            if engine is None:
-                engine = Engine(device)
-            res, _, _ = engine.batch_exact(inst)
-            vals = [r.value for r in res]
-            for c, norm, (snum, sden) in pending:
-                num = norm * norm if snum is None else vals[snum]
-                den = norm * norm if sden is None else vals[sden]
-                out[c] = 0.0 if num == 0 else 2.0 ** (v - uu) * num / den
+        shots_r = PH.shape[0]
+        out = np.zeros(shots_r)
+        base = 2.0 ** (v - uu)
+        g = len(xk)
+        # probability.py:140-150: the first len(G') generators of H' agree
+        if g <= len(Hm[0]) and xk == Hm[1][:g] and zk == Hm[2][:g]:
+            same = (PH == np.asarray(Hm[0][:g], dtype=np.int64)[None, :]).all(axis=1)
+        else:
+            same = np.zeros(shots_r, dtype=bool)
+        out[same & ~impossible] = base
+        todo = np.nonzero(~same & ~impossible)[0]
+        if len(todo) == 0:
+            return out
+        # G' is not empty here (an empty G' agrees with H'); an empty H'
+        # needs the exact decomposition's norm
+        callcfg = dict(cfg)
+        callcfg["exact"] = bool(exact) or len(Hm[0]) == 0
+        seeds = [int(sd) for sd in rand_int(2 ** 62, len(todo))]
+        Ls, norm, ex, k = decompose_many(t, callcfg, seeds)
+        if t == 0 or sampling_kept(callcfg.get("noapprox"), callcfg.get("forceSample"), ex, t, k,
+                                   int(callcfg.get("samples") or 0), int(callcfg.get("bins") or 0)):
+            # sampled norms, or the t = 0 closed form: one call each
+            for c, sd in zip(todo, seeds):
+                callcfg["seed"] = sd
+                num, den = projector_norms(_bits((list(PH[c]), xk, zk), t), _bits(Hm, t), t, callcfg)
+                out[c] = 0.0 if num == 0 else base * num / den
+            return out
+        if ex:
+            # no randomness: one record per distinct G', one for H'
+            uniq, inv = np.unique(PH[todo], axis=0, return_inverse=True)
+            inv = inv.reshape(-1)
+            nnum = len(uniq)
+            A = instance_array(nnum + (1 if Hm[0] else 0))
+            fill(A, slice(0, nnum), uniq, xk, zk, True, k, None)
+            if Hm[0]:
+                fill(A, slice(nnum, nnum + 1), [Hm[0]], Hm[1], Hm[2], True, k, None)
+        else:
+            nnum = len(todo)
+            inv = np.arange(nnum)
+            A = instance_array(nnum * (2 if Hm[0] else 1))
+            fill(A, slice(0, nnum), PH[todo], xk, zk, False, k, Ls)
+            if Hm[0]:
+                fill(A, slice(nnum, 2 * nnum), [Hm[0]], Hm[1], Hm[2], False, k, Ls)
+        if engine is None:
+            engine = Engine(device)
+        res, _, _ = engine.batch_exact(A)
+        vals = np.array([r.value for r in res])
+        num = vals[:nnum][inv]
+        if not Hm[0]:
+            den = np.full(len(todo), norm * norm)
+        elif ex:
+            den = np.full(len(todo), vals[nnum])
+        else:
+            den = vals[nnum:]
+        out[todo] = np.where(num == 0, 0.0, base * num / np.where(num == 0, 1.0, den))
         return out
 
     def round_projectors(m, bits):
         """G' of every shot when the first m generators of order are set;
-        bits[s] holds shot s's outcomes for them."""
+        bits (shots x m) holds the shots' outcomes for them."""
         rows = (list(G[0][:m]) + list(G[0][nq:]), list(G[1][:m]) + list(G[1][nq:]),
                 list(G[2][:m]) + list(G[2][nq:]))
         prods = _products(n, rows)
-        pad = len(rows[0]) - m
-        return [_shot_projector(prods, np.array(list(b) + [0] * pad, dtype=int)) for b in bits], prods[4]
+        V = np.zeros((bits.shape[0], len(rows[0])), dtype=np.int64)
+        V[:, :m] = bits
+        return _round_phases(prods, V), prods[4]
 
     try:
         ncon = len(measure)
-        con_bits = [int(measure[q]) for q in order[:ncon]]
+        con_bits = np.array([int(measure[q]) for q in order[:ncon]], dtype=np.int64)
         Psofar = np.ones(shots)
         if ncon:
-            calls, uu = round_projectors(ncon, [con_bits])
-            P = evaluate(calls, uu, True)[0]
+            (PH, xk, zk, imp), uu = round_projectors(ncon, con_bits[None, :])
+            P = evaluate(PH, xk, zk, imp, uu, True)[0]
             if P == 0:
                 return [""] * shots
             if P < 1e-5:
                 print("Warning: probability of satisfying qubit constraints very low (%f). Could be impossible." % P)
             Psofar[:] = P
-        bits = [list(con_bits) for _ in range(shots)]
-        out = [""] * shots
+        bits = np.zeros((shots, ncon + len(sample)), dtype=np.int64)
+        bits[:, :ncon] = con_bits
         for r in range(len(sample)):
-            calls, uu = round_projectors(ncon + r + 1, [b + [0] for b in bits])
-            P = np.array(evaluate(calls, uu, cfg["exact"]))
+            m = ncon + r + 1
+            (PH, xk, zk, imp), uu = round_projectors(m, bits[:, :m])
+            P = evaluate(PH, xk, zk, imp, uu, cfg["exact"])
             P0 = P / Psofar
             u01 = rng.random(shots) if rng is not None else np.random.random(shots)
             draw = np.where(u01 < P0, 0, 1)
-            for s in range(shots):
-                bits[s].append(int(draw[s]))
-                out[s] += str(int(draw[s]))
+            bits[:, m - 1] = draw
             Psofar = _next_psofar(cfg.get("chain", "reference"), P, P0, Psofar, draw)
-        return out
+        return ["".join(str(int(b)) for b in row[ncon:]) for row in bits]
     finally:
         if engine is not None:
             engine.close()

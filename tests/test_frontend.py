@@ -415,6 +415,35 @@ def test_batched_truncation_matches_truncate():
                 assert got == (want[0], want[1], want[2])
 
 
+def test_round_phases_match_shot_projector():
+    # every shot's G' at once equals _shot_projector shot by shot
+    from circuitsimulator_amd.sample import _products, _round_phases, _shot_projector
+    rng = np.random.default_rng(13)
+    for trial in range(6):
+        n = int(rng.integers(3, 6))
+        while True:
+            circ = random_clifford_t(rng, n, int(rng.integers(8, 20)), int(rng.integers(1, 6)))
+            if valid_ts(circ):
+                break
+        t = gadgetize.countY(circ)[0]
+        y = [int(b) for b in rng.integers(0, 2, size=t)]
+        order = [int(q) for q in rng.permutation(n)[: int(rng.integers(1, n + 1))]]
+        G = gadgetize.gadgetize(circ, {q: 0 for q in order}, y)
+        for m in range(1, len(order) + 1):
+            rows = (G[0][:m] + G[0][len(order):], G[1][:m] + G[1][len(order):], G[2][:m] + G[2][len(order):])
+            prods = _products(n, rows)
+            V = np.zeros((16, len(rows[0])), dtype=np.int64)
+            V[:, :m] = rng.integers(0, 2, size=(16, m))
+            PH, xk, zk, imp = _round_phases(prods, V)
+            for s in range(16):
+                want = _shot_projector(prods, V[s])
+                if want is None:
+                    assert imp[s]
+                    continue
+                assert not imp[s]
+                assert ([int(p) for p in PH[s]], xk, zk) == (want[0], want[1], want[2])
+
+
 def test_batched_shots_clifford_on_cpu():
     # t = 0 uses the closed form: no GPU needed.  GHZ: the joint chain gives
     # only 000 and 111; the reference's chain (sample.py:54,84) divides the
