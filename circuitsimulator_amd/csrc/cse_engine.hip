@@ -35,7 +35,8 @@ struct cse_ctx {
     char err[512] = {0};
     DevBuf s1, s2, out, recs, terms, zsum, kahan, hist, ips, queue, itemoff, inst, projs, thr, sexp, L,
         idx, errflag, recoff, lsh;
-    HostBuf hstage;  // per-sample sums of cse_sampled_projector
+    HostBuf hstage;  // results coming back (per-sample sums, histograms)
+    HostBuf hup;     // the plan going up
     bool thr_ready = false;
     int blocks_per_cu = 8;  // persistent grid: blocks of 4 waves per CU
 };
@@ -174,19 +175,29 @@ struct Plan {
     long long nrec = 0, nitems = 0, nterms = 0;
 };
 
+// The plan goes up through one pinned buffer (single DMA transfers); every
+// call synchronizes its stream before returning, so the buffer is free
+// again at the next call.
 int upload_plan(cse_ctx *ctx, const Plan &pl) {
     const int ninst = (int)pl.inst.size();
     ENSURE(inst, sizeof(InstDesc) * ninst);
     ENSURE(itemoff, sizeof(long long) * (ninst + 1));
     ENSURE(recoff, sizeof(long long) * (ninst + 1));
     ENSURE(projs, sizeof(DevProjector) * ninst);
-    HIPCHK(hipMemcpyAsync(ctx->inst.p, pl.inst.data(), sizeof(InstDesc) * ninst, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->itemoff.p, pl.item_off.data(), sizeof(long long) * (ninst + 1),
-                          hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->recoff.p, pl.rec_off.data(), sizeof(long long) * (ninst + 1),
-                          hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->projs.p, pl.projs.data(), sizeof(DevProjector) * ninst, hipMemcpyHostToDevice,
-                          ctx->stream));
+    const size_t sz[4] = {sizeof(InstDesc) * ninst, sizeof(long long) * (ninst + 1),
+                          sizeof(long long) * (ninst + 1), sizeof(DevProjector) * ninst};
+    const void *src[4] = {pl.inst.data(), pl.item_off.data(), pl.rec_off.data(), pl.projs.data()};
+    void *dst[4] = {ctx->inst.p, ctx->itemoff.p, ctx->recoff.p, ctx->projs.p};
+    size_t total = 0;
+    for (int b = 0; b < 4; b++) total += (sz[b] + 255) & ~(size_t)255;
+    int rc = ensure_host(ctx, ctx->hup, total);
+    if (rc) return rc;
+    char *h = (char *)ctx->hup.p;
+    for (int b = 0; b < 4; b++) {
+        memcpy(h, src[b], sz[b]);
+        HIPCHK(hipMemcpyAsync(dst[b], h, sz[b], hipMemcpyHostToDevice, ctx->stream));
+        h += (sz[b] + 255) & ~(size_t)255;
+    }
     return CSE_OK;
 }
 
@@ -333,6 +344,7 @@ void cse_destroy(cse_ctx *ctx) {
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->hstage.p) (void)hipHostFree(ctx->hstage.p);
+    if (ctx->hup.p) (void)hipHostFree(ctx->hup.p);
     if (ctx->e0) (void)hipEventDestroy(ctx->e0);
     if (ctx->e1) (void)hipEventDestroy(ctx->e1);
     if (ctx->e2) (void)hipEventDestroy(ctx->e2);
@@ -547,18 +559,24 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
     rc = run_sweep(ctx, pl, 1, terms_out != nullptr);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
-    std::vector<unsigned long long> hist((size_t)HIST_STRIDE * ninst);
-    std::vector<double> kz(2 * (size_t)ninst);
-    std::vector<unsigned long long> nips((size_t)ninst);
-    int errv = 0;
-    HIPCHK(hipMemcpyAsync(&errv, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(hist.data(), ctx->hist.p, 8 * hist.size(), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(kz.data(), ctx->kahan.p, 16 * (size_t)ninst, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(nips.data(), ctx->ips.p, 8 * (size_t)ninst, hipMemcpyDeviceToHost, ctx->stream));
+    // one pinned block: histograms (HIST_STRIDE words per instance), the
+    // Kahan pairs, the inner-product counts, the error flag
+    const size_t ni = (size_t)ninst, nh = (size_t)HIST_STRIDE * ni;
+    rc = ensure_host(ctx, ctx->hstage, 8 * nh + 16 * ni + 8 * ni + 8);
+    if (rc) return rc;
+    char *hs = (char *)ctx->hstage.p;
+    const unsigned long long *hist = (const unsigned long long *)hs;
+    const double *kz = (const double *)(hs + 8 * nh);
+    const unsigned long long *nips = (const unsigned long long *)(hs + 8 * nh + 16 * ni);
+    int *errp = (int *)(hs + 8 * nh + 24 * ni);
+    HIPCHK(hipMemcpyAsync(errp, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hs, ctx->hist.p, 8 * nh, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hs + 8 * nh, ctx->kahan.p, 16 * ni, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hs + 8 * nh + 16 * ni, ctx->ips.p, 8 * ni, hipMemcpyDeviceToHost, ctx->stream));
     if (terms_out)
         HIPCHK(hipMemcpyAsync(terms_out, ctx->terms.p, 4 * (size_t)pl.nterms, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (errv & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
+    if (*errp & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
     const double kms = elapsed(ctx->e1, ctx->e2), tms = elapsed(ctx->e0, ctx->e2);
     if (kernel_ms) *kernel_ms = kms;
     for (int i = 0; i < ninst; i++) {

@@ -129,42 +129,54 @@ static const cse_dd SQRT2 = {1.4142135623730951, -9.667293313452913e-17};
 /* ------------------------------------------------ exact -> FP64 */
 void cse_hist_value(const int64_t *hist, int pmin, double *re, double *im)
 {
-    /* bin (P, M), P = -(b/8): count * 2^{P/2} w^M = count * 2^e * (1 or sqrt2) * (+-1 or +-i);
-     * w^M for odd M carries 1/sqrt2. */
-    cse_dd xr = {0, 0}, yr = {0, 0}, xi = {0, 0}, yi = {0, 0};
-    const int nb = (1 - pmin) * 8;
-    for (int b = 0; b < nb; b++) {
-        int64_t c = hist[b];
-        if (!c) continue;
-        const int P = -(b / 8), M = b % 8;
-        /* 2^{P/2} = 2^{floor(P/2)} sqrt2^{P&1}; odd M: (+-1 +- i)/sqrt2 */
-        const int podd = P & 1;
-        int e = P >> 1;
-        int rs, is;  /* signs of the real / imaginary unit parts */
-        static const int cre[8] = {1, 1, 0, -1, -1, -1, 0, 1};
-        static const int cim[8] = {0, 1, 1, 1, 0, -1, -1, -1};
-        rs = cre[M];
-        is = cim[M];
-        /* factor = sqrt2^{podd} * (M odd ? 1/sqrt2 : 1) = sqrt2^{podd - modd} */
-        int s2 = podd - (M & 1); /* -1, 0 or 1 */
-        int useq = 0;
-        if (s2 == -1) { e -= 1; useq = 1; } /* 1/sqrt2 = sqrt2 / 2 */
-        else if (s2 == 1) useq = 1;
-        cse_dd v = cse_dd_from_i128((__int128)c);
-        v = dd_scale(v, e);
-        cse_dd vr = {rs * v.hi, rs * v.lo}, vi = {is * v.hi, is * v.lo};
-        if (useq) {
-            yr = cse_dd_add(yr, vr);
-            yi = cse_dd_add(yi, vi);
-        } else {
-            xr = cse_dd_add(xr, vr);
-            xi = cse_dd_add(xi, vi);
-        }
+    /* bin (P, M) = hist[8 (-P) + M], P in [pmin, 0]: count * 2^{P/2} w^M.
+     * With 2^{P/2} = 2^{floor(P/2)} sqrt2^{P&1} and w^M = (cre + i cim) /
+     * sqrt2^{M&1}, a row P puts its even-M bins at 2^{floor(P/2)} times
+     * sqrt2^{P&1} and its odd-M bins at 2^{floor(P/2) - (1 - P&1)} times
+     * sqrt2^{1 - P&1}.  The counts are summed exactly per (power of two,
+     * sqrt2 or not, real or imaginary) in __int128 (at most 2^63 per bin),
+     * and each non-empty sum is converted once. */
+    static const int cre[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+    static const int cim[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+    const int rows = 1 - pmin;
+    const int emin = (pmin >> 1) - 1; /* powers of two in [emin, 0] */
+    enum { NE = 66 };
+    if (emin < -(NE - 1)) {
+        *re = *im = NAN;
+        return;
     }
-    cse_dd r = cse_dd_add(xr, cse_dd_mul(yr, SQRT2));
-    cse_dd i = cse_dd_add(xi, cse_dd_mul(yi, SQRT2));
+    __int128 acc[4][NE]; /* 0: re, 1: im, 2: re * sqrt2, 3: im * sqrt2 */
+    unsigned char used[NE] = {0};
+    for (int row = 0; row < rows; row++) {
+        const int64_t *h = hist + 8 * row;
+        if (!(h[0] | h[1] | h[2] | h[3] | h[4] | h[5] | h[6] | h[7])) continue;
+        const int P = -row, podd = P & 1;
+        /* even M: (c0 - c4) + (c2 - c6) i; odd M: (c1 - c3 - c5 + c7) + (c1 + c3 - c5 - c7) i */
+        const __int128 er = (__int128)h[0] - h[4], ei = (__int128)h[2] - h[6];
+        __int128 orr = 0, oi = 0;
+        for (int M = 1; M < 8; M += 2) {
+            orr += (__int128)h[M] * cre[M];
+            oi += (__int128)h[M] * cim[M];
+        }
+        const int e0 = (P >> 1) - emin, q0 = podd ? 2 : 0;
+        const int e1 = (P >> 1) - (podd ? 0 : 1) - emin, q1 = podd ? 0 : 2;
+        if (!used[e0]) { used[e0] = 1; acc[0][e0] = acc[1][e0] = acc[2][e0] = acc[3][e0] = 0; }
+        if (!used[e1]) { used[e1] = 1; acc[0][e1] = acc[1][e1] = acc[2][e1] = acc[3][e1] = 0; }
+        acc[q0][e0] += er;
+        acc[q0 + 1][e0] += ei;
+        acc[q1][e1] += orr;
+        acc[q1 + 1][e1] += oi;
+    }
+    cse_dd s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int i = 0; i < NE; i++) {
+        if (!used[i]) continue;
+        for (int q = 0; q < 4; q++)
+            if (acc[q][i]) s[q] = cse_dd_add(s[q], dd_scale(cse_dd_from_i128(acc[q][i]), i + emin));
+    }
+    cse_dd r = cse_dd_add(s[0], cse_dd_mul(s[2], SQRT2));
+    cse_dd im_ = cse_dd_add(s[1], cse_dd_mul(s[3], SQRT2));
     *re = r.hi + r.lo;
-    *im = i.hi + i.lo;
+    *im = im_.hi + im_.lo;
 }
 
 /* (hi * 2^64 + lo) * 2^e in double-double; the carry of lo joins hi

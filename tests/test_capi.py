@@ -97,3 +97,37 @@ def test_sampled_estimator_sum_is_exact():
     want = (D(X.numerator) / D(X.denominator) + D(2).sqrt() * D(Y.numerator) / D(Y.denominator)) / count
     assert abs(D(mean.value) - want) <= abs(want) * D(2) ** -52
     assert mean_k.value == 0.0
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libcse.so not built")
+def test_hist_value_is_exact():
+    """cse_hist_value: sum over bins (P, M) of count * 2^{P/2} w^M, rounded
+    once.  Checked against rational arithmetic (the sqrt2 parts at 60
+    digits) on random sparse histograms with signed counts up to 2^62."""
+    import decimal
+    from fractions import Fraction
+    lib = ctypes.CDLL(LIB)
+    np = __import__("numpy")
+    decimal.getcontext().prec = 60
+    D = decimal.Decimal
+    r2 = D(2).sqrt()
+    rng = np.random.default_rng(2)
+    cre, cim = [1, 1, 0, -1, -1, -1, 0, 1], [0, 1, 1, 1, 0, -1, -1, -1]
+    for trial in range(60):
+        H = np.zeros(129 * 8, dtype=np.int64)
+        idx = rng.choice(len(H), size=int(rng.integers(1, 200)), replace=False)
+        H[idx] = rng.integers(-(1 << 62), 1 << 62, size=len(idx)) >> int(rng.integers(0, 60))
+        re, im = ctypes.c_double(), ctypes.c_double()
+        lib.cse_hist_value(H.ctypes.data_as(ctypes.c_void_p), -128, ctypes.byref(re), ctypes.byref(im))
+        X = [Fraction(0)] * 4  # re, im, re * sqrt2, im * sqrt2
+        for b in np.nonzero(H)[0]:
+            b = int(b)
+            c, P, M = int(H[b]), -(b // 8), b % 8
+            # 2^{P/2} w^M = 2^{(P - (M & 1)) / 2} (cre + i cim)
+            x = P - (M & 1)
+            w, q = Fraction(2) ** (x >> 1), 2 * (x & 1)
+            X[q] += c * cre[M] * w
+            X[q + 1] += c * cim[M] * w
+        for got, a, s in ((re.value, X[0], X[2]), (im.value, X[1], X[3])):
+            want = D(a.numerator) / D(a.denominator) + r2 * D(s.numerator) / D(s.denominator)
+            assert abs(D(got) - want) <= abs(want) * D(2) ** -52 + D(2) ** -1074
