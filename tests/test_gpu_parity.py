@@ -277,6 +277,30 @@ def test_sampled_t64_unpacked_layout(E):
     assert np.array_equal(ex["zsum"], zsum)
 
 
+@pytest.mark.parametrize("t", [60, 61])
+def test_sampled_layout_limits(E, t):
+    # n = 60 is the packed layout's limit (row 60 is the loop's end row);
+    # n = 61 runs the unpacked sweep
+    from circuitsimulator_amd.engine import Projector
+    r, k = 12, 6
+    rng = np.random.default_rng(600 + t)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    L = [orc.rand_bits(rng, t) for _ in range(k)]
+    res, ex = E.sampled_projector(Projector.make(ph, xs, zs, t), L, k, 0, 5, 0, 6, want_terms=True)
+    Po = orc.make_projector(ph, xs, zs, t)
+    thr = orc.thresholds(t)
+    phis = [orc.prepL(j, t, L) for j in range(1 << k)]
+    for s in range(6):
+        th = orc.random_state(t, 5, s, thr)
+        e = orc.project(th, Po)
+        assert ex["sexp"][s] == e
+        if e < 0:
+            continue
+        for j in range(1 << k):
+            eps, p, m = orc.inner(th, phis[j])
+            assert ex["terms"][s][j] == pack_term(eps, p, m), (s, j)
+
+
 def test_sampled_t48_workload_subsample(E):
     # the benchmark's workload shape: t=48, r=24, k=12 (chi=4096)
     from circuitsimulator_amd.workloads import sampled_config
