@@ -12,6 +12,11 @@
 //                    every wave computes <phi_j | theta'> for the chunk's j,
 //                    phi_j generated on the fly from (L, j) or (t, j), and
 //                    accumulates exactly (int64) plus a Kahan-FP32 shadow.
+//   k_sweep_packed   the same with D packed into the J rows (n <= 60); the
+//                    |L> terms of records with at most two non-pivot columns
+//                    run 64 at a time in one asm block (phiL_batch)
+//   k_lspace_hist    histogram of |x| over the row space of L (decompose's
+//                    fidelity)
 #include "cse_device.h"
 #include "cse_internal.h"
 
