@@ -277,6 +277,24 @@ def test_sampled_t64_unpacked_layout(E):
     assert np.array_equal(ex["zsum"], zsum)
 
 
+@pytest.mark.parametrize("chunk", [100, 192, 64])
+def test_sampled_chunks_across_batches(E, chunk):
+    # chi = 256 split into items of `chunk` j: items starting at a multiple
+    # of 64 build x_j from a low-bit lane table and a high part advanced
+    # between batches; the others build it per lane
+    from circuitsimulator_amd.engine import Projector
+    t, r, k = 30, 8, 8
+    rng = np.random.default_rng(700 + chunk)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    L = [orc.rand_bits(rng, t) for _ in range(k)]
+    chi = 1 << k
+    res, ex = E.sampled_projector(Projector.make(ph, xs, zs, t), L, k, 0, 11, 2, 4, chunk=chunk, want_terms=True)
+    se, zsum, terms = oracle_sampled(t, k, L, 0, ph, xs, zs, 11, 2, 4, chi)
+    assert np.array_equal(ex["sexp"], se)
+    assert np.array_equal(ex["terms"], terms)
+    assert np.array_equal(ex["zsum"], zsum)
+
+
 @pytest.mark.parametrize("t", [60, 61])
 def test_sampled_layout_limits(E, t):
     # n = 60 is the packed layout's limit (row 60 is the loop's end row);
