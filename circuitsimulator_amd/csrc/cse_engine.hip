@@ -9,6 +9,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "../../include/cse.h"
@@ -579,19 +581,36 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
     if (*errp & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
     const double kms = elapsed(ctx->e1, ctx->e2), tms = elapsed(ctx->e0, ctx->e2);
     if (kernel_ms) *kernel_ms = kms;
-    for (int i = 0; i < ninst; i++) {
-        cse_exact_result *r = &res[i];
-        const int64_t *h = (const int64_t *)&hist[(size_t)HIST_STRIDE * i];
-        r->pmin = -128;
-        if (r->hist) memcpy(r->hist, h, 8 * (size_t)HIST_STRIDE);
-        cse_hist_value(h, -128, &r->re, &r->im);
-        r->value = hypot(r->re, r->im);
-        r->kahan_re = kz[2 * i];
-        r->kahan_im = kz[2 * i + 1];
-        r->terms = pl.inst[i].chi * (pl.inst[i].chi + 1) / 2;
-        r->ips = (int64_t)nips[i];
-        r->kernel_ms = kms;
-        r->total_ms = tms;
+    // each instance's exact value from its histogram (a few microseconds
+    // each): split over host threads for large batches
+    auto convert = [&](int lo, int hi) {
+        for (int i = lo; i < hi; i++) {
+            cse_exact_result *r = &res[i];
+            const int64_t *h = (const int64_t *)&hist[(size_t)HIST_STRIDE * i];
+            r->pmin = -128;
+            if (r->hist) memcpy(r->hist, h, 8 * (size_t)HIST_STRIDE);
+            cse_hist_value(h, -128, &r->re, &r->im);
+            r->value = hypot(r->re, r->im);
+            r->kahan_re = kz[2 * i];
+            r->kahan_im = kz[2 * i + 1];
+            r->terms = pl.inst[i].chi * (pl.inst[i].chi + 1) / 2;
+            r->ips = (int64_t)nips[i];
+            r->kernel_ms = kms;
+            r->total_ms = tms;
+        }
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nthr = ninst >= 512 ? (int)std::min(8u, hw ? hw : 1u) : 1;
+    if (nthr > 1) {
+        std::vector<std::thread> pool;
+        const int per = (ninst + nthr - 1) / nthr;
+        for (int w = 0; w < nthr; w++) {
+            const int lo = w * per, hi = std::min(ninst, lo + per);
+            if (lo < hi) pool.emplace_back(convert, lo, hi);
+        }
+        for (std::thread &th : pool) th.join();
+    } else {
+        convert(0, ninst);
     }
     return CSE_OK;
 }
