@@ -327,6 +327,40 @@ def test_c1_on_engine():
 
 
 @pytest.mark.gpu
+def test_c5_circuit_probabilities_against_oracle():
+    # the sampling workload's circuit (n=40, 400 {H,S,CX} + 40 T, t=40) with
+    # an |L> of k=5: probability() on the engine against the oracle's exact
+    # pair sums of the same G', H' and L (same y, same decompose seed)
+    from circuitsimulator_amd.probability import decompose, probability
+    from circuitsimulator_amd.workloads import c1_circuit
+    from gpu_util import oracle_exact_terms, term_value
+    circ, _ = c1_circuit(seed=1005, n=40, gates=400, ts=40)
+    t = gadgetize.countY(circ)[0]
+    y = "".join(str(int(b)) for b in np.random.default_rng(40).integers(0, 2, size=t))
+    for measure in ({0: 0}, {0: 1, 1: 0}):
+        cfg = {"exact": False, "k": 5, "quiet": True, "y": y, "x": "", "seed": 77}
+        got = probability(circ, dict(measure), config=dict(cfg))
+        G, H, n, _t = projectors.projectors(circ, dict(measure), x="", y=y)
+        Gp, u = projectors.truncate(n, G)
+        Hp, v = projectors.truncate(n, H)
+        L, norm, ex, k = decompose(t, dict(cfg, fidbound=1e-5), 77)
+        assert (ex, k) == (False, 5)
+
+        def oracle_norm(P):
+            ph, xs, zs = masks(P)
+            tot = 0
+            for (pt, s, i, j) in oracle_exact_terms(t, k, L, 0, ph, xs, zs):
+                if pt:
+                    w = term_value(pt, s)
+                    tot += w if i == j else 2 * w.real
+            return abs(tot)
+
+        num, den = oracle_norm(Gp), oracle_norm(Hp)
+        want = 0.0 if num == 0 else 2.0 ** (v - u) * num / den
+        assert got == pytest.approx(want, rel=1e-12, abs=1e-15), measure
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("seed", range(4))
 def test_probability_exact_on_engine(seed):
     from circuitsimulator_amd.probability import probability
