@@ -173,6 +173,31 @@ DEV void rs_prepL(RState &s, u64 i, int t, const uint64_t *L, int k) {
         if (!((xv >> q) & 1ull)) rs_shrink(s, bit(q), 0, false);
 }
 
+// The same state as rs_prepL, K = span{e_q : q in supp(x_i)}, h = 0, no
+// phase, built directly for records (whose later reduction to echelon form
+// makes the row order irrelevant): rows 0..k-1 are the unit vectors of the
+// support in increasing q, rows k..t-1 the others, so G is a permutation
+// and Gb = G.  Lane q sends q to the row r(q) it lands in.
+DEV void rs_prepL_direct(RState &s, u64 i, int t, const uint64_t *L, int k) {
+    const int x = lane_id();
+    u64 xv = 0;
+    for (int j = 0; j < k; j++)
+        if ((i >> (k - 1 - j)) & 1ull) xv ^= L[j];
+    const u64 below = lowmask(x);
+    const int kk = __popcll(xv);
+    int r = x;
+    if (x < t) r = ((xv >> x) & 1ull) ? __popcll(xv & below) : kk + __popcll(~xv & lowmask(t) & below);
+    const int q = __builtin_amdgcn_ds_permute(r << 2, x);
+    s.G = x < t ? bit(q) : 0ull;
+    s.Gb = s.G;
+    s.J = 0;
+    s.d = 0;
+    s.n = t;
+    s.k = kk;
+    s.Q = 0;
+    s.h = 0;
+}
+
 // stateprep.c:36-81 in the reference row bookkeeping
 DEV void rs_prepH(RState &s, u64 i, int t) {
     const int x = lane_id();
@@ -327,7 +352,7 @@ k_records(int mode, int ninst, const long long *__restrict__ rec_off, const Inst
     } else if (D.exact) {
         rs_prepH(s, (u64)local, n);
     } else {
-        rs_prepL(s, (u64)local, n, D.L, D.k);
+        rs_prepL_direct(s, (u64)local, n, D.L, D.k);
     }
     const int se = rs_project(s, projs[in]);
     if (lane_id() == 0) sexp_out[g] = se;
