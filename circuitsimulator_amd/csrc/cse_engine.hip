@@ -203,7 +203,7 @@ int upload_plan(cse_ctx *ctx, const Plan &pl) {
     return CSE_OK;
 }
 
-int run_records(cse_ctx *ctx, const Plan &pl, int mode, uint64_t seed, long long first) {
+int run_records(cse_ctx *ctx, const Plan &pl, int mode) {
     const int ninst = (int)pl.inst.size();
     ENSURE(recs, (size_t)REC_WORDS * 8 * (size_t)pl.nrec);
     ENSURE(sexp, 4 * (size_t)pl.nrec);
@@ -215,7 +215,7 @@ int run_records(cse_ctx *ctx, const Plan &pl, int mode, uint64_t seed, long long
     if (blocks > 0) {
         hipLaunchKernelGGL(k_records, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, mode, ninst,
                            dp<long long>(ctx->recoff), dp<InstDesc>(ctx->inst), dp<DevProjector>(ctx->projs),
-                           dp<uint64_t>(ctx->thr), seed, first, pl.nrec, dp<uint64_t>(ctx->recs),
+                           dp<uint64_t>(ctx->thr), pl.nrec, dp<uint64_t>(ctx->recs),
                            dp<int32_t>(ctx->sexp), dp<int>(ctx->errflag));
         HIPCHK(hipGetLastError());
     }
@@ -226,7 +226,8 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     const int ninst = (int)pl.inst.size();
     ENSURE(queue, 8);
     ENSURE(ips, 8 * (size_t)ninst);
-    ENSURE(hist, 8 * (size_t)HIST_STRIDE * ninst);
+    // the histograms are those of exact mode; sampled mode sums per record
+    if (mode == 1) ENSURE(hist, 8 * (size_t)HIST_STRIDE * ninst);
     if (mode == 0) {
         ENSURE(zsum, 32 * (size_t)pl.nrec);
         ENSURE(kahan, 16 * (size_t)pl.nrec);
@@ -240,7 +241,7 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     }
     int rc = zero64(ctx, ctx->queue.p, 1);
     if (!rc) rc = zero64(ctx, ctx->ips.p, ninst);
-    if (!rc) rc = zero64(ctx, ctx->hist.p, (long long)HIST_STRIDE * ninst);
+    if (!rc && mode == 1) rc = zero64(ctx, ctx->hist.p, (long long)HIST_STRIDE * ninst);
     if (rc) return rc;
     if (want_terms) ENSURE(terms, 4 * (size_t)(pl.nterms > 0 ? pl.nterms : 1));
     SweepArgs A;
@@ -254,7 +255,7 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     A.terms = want_terms ? dp<int32_t>(ctx->terms) : nullptr;
     A.zsum = mode == 0 ? dp<long long>(ctx->zsum) : nullptr;
     A.kahan = dp<double>(ctx->kahan);
-    A.hist = dp<unsigned long long>(ctx->hist);
+    A.hist = mode == 1 ? dp<unsigned long long>(ctx->hist) : nullptr;
     A.hist_stride = HIST_STRIDE;
     A.ips = dp<unsigned long long>(ctx->ips);
     A.err = dp<int>(ctx->errflag);
@@ -286,6 +287,8 @@ InstDesc make_inst(int t, int k, int exact, const uint64_t *L, int chunk) {
 // most 2^32 in each component at the 2^32 fixed-point scale, so chi terms stay
 // below 2^62.  (The reference holds chi in an int, innerprod.c:122-131.)
 constexpr int kMaxLogChi = 30;
+// samples (records) per sampled-mode call
+constexpr long long kMaxSamples = 1ll << 24;
 
 int check_decomp(cse_ctx *ctx, int t, int k, int exact, const uint64_t *L) {
     if (exact) {
@@ -464,73 +467,140 @@ int cse_random_states(cse_ctx *ctx, int n, uint64_t seed, int64_t first, size_t 
     return CSE_OK;
 }
 
-int cse_sampled_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k, int exact,
-                          uint64_t seed, int64_t first, int64_t count, int chunk, cse_sampled_result *res) {
-    if (!ctx || !res) return CSE_ERR_ARG;
-    int rc = check_projector(ctx, P);
-    if (rc) return rc;
-    const int t = P->nqubits;
-    rc = check_decomp(ctx, t, k, exact, L);
-    if (rc) return rc;
-    if (count < 1) return fail(ctx, CSE_ERR_ARG, "count=%lld", (long long)count);
-    if (count > (1ll << 24)) return fail(ctx, CSE_ERR_ARG, "count=%lld above 2^24 per call", (long long)count);
-    HIPCHK(hipSetDevice(ctx->device));
-    rc = ensure_thresholds(ctx);
-    if (rc) return rc;
-
+// Sampled-mode instances [0, ninst): one record launch and one sweep over all
+// of them, then each instance's estimator from its own samples.
+static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst, int chunk,
+                         cse_sampled_result *res, double *kernel_ms) {
     Plan pl;
-    pl.inst.push_back(make_inst(t, k, exact, L, chunk));
-    pl.projs.push_back(to_dev(P));
-    pl.rec_off = {0, count};
-    pl.nrec = count;
-    pl.nitems = count * pl.inst[0].nchunks;
-    pl.item_off = {0, pl.nitems};
-    pl.nterms = count * pl.inst[0].chi;
-    if (res->terms && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
-    rc = upload_plan(ctx, pl);
+    pl.rec_off.push_back(0);
+    pl.item_off.push_back(0);
+    bool want_terms = false;
+    for (int i = 0; i < ninst; i++) {
+        InstDesc d = make_inst(in[i].P.nqubits, in[i].k, in[i].exact, in[i].L, chunk);
+        d.seed = in[i].seed;
+        d.first = in[i].first;
+        d.rec_off = pl.nrec;
+        d.term_off = pl.nterms;
+        pl.nrec += in[i].count;
+        pl.nitems += in[i].count * d.nchunks;
+        pl.nterms += in[i].count * d.chi;
+        pl.rec_off.push_back(pl.nrec);
+        pl.item_off.push_back(pl.nitems);
+        pl.inst.push_back(d);
+        pl.projs.push_back(to_dev(&in[i].P));
+        want_terms = want_terms || res[i].terms != nullptr;
+    }
+    if (pl.nrec > kMaxSamples) return fail(ctx, CSE_ERR_ARG, "%lld samples above 2^24 per call", pl.nrec);
+    if (want_terms && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
+    int rc = ensure_thresholds(ctx);
+    if (!rc) rc = upload_plan(ctx, pl);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
-    rc = run_records(ctx, pl, 0, seed, first);
+    rc = run_records(ctx, pl, 0);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e1, ctx->stream));
-    rc = run_sweep(ctx, pl, 0, res->terms != nullptr);
+    rc = run_sweep(ctx, pl, 0, want_terms);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
 
-    // one pinned block: z (32 B/sample), kz (16), sexp (4), then ips and the error flag
-    const size_t n = (size_t)count;
-    rc = ensure_host(ctx, ctx->hstage, 52 * n + 16);
+    // one pinned block: z (32 B/sample), kz (16), sexp (4), then the
+    // per-instance inner-product counts and the error flag
+    const size_t n = (size_t)pl.nrec, ni = (size_t)ninst, oips = (52 * n + 7) & ~(size_t)7;
+    rc = ensure_host(ctx, ctx->hstage, oips + 8 * ni + 8);
     if (rc) return rc;
     char *hs = (char *)ctx->hstage.p;
     const long long *z = (const long long *)hs;
     const double *kz = (const double *)(hs + 32 * n);
     const int32_t *se = (const int32_t *)(hs + 48 * n);
-    unsigned long long *nipsp = (unsigned long long *)(hs + 52 * n);
-    int *errp = (int *)(hs + 52 * n + 8);
+    const unsigned long long *nips = (const unsigned long long *)(hs + oips);
+    int *errp = (int *)(hs + oips + 8 * ni);
     HIPCHK(hipMemcpyAsync(hs, ctx->zsum.p, 32 * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(hs + 32 * n, ctx->kahan.p, 16 * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(hs + 48 * n, ctx->sexp.p, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(nipsp, ctx->ips.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hs + oips, ctx->ips.p, 8 * ni, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(errp, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    if (res->terms)
-        HIPCHK(hipMemcpyAsync(res->terms, ctx->terms.p, 4 * (size_t)pl.nterms, hipMemcpyDeviceToHost, ctx->stream));
+    for (int i = 0; i < ninst; i++)
+        if (res[i].terms)
+            HIPCHK(hipMemcpyAsync(res[i].terms, dp<int32_t>(ctx->terms) + pl.inst[i].term_off,
+                                  4 * (size_t)(in[i].count * pl.inst[i].chi), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    const unsigned long long nips = *nipsp;
     const int errv = *errp;
     if (errv & 1) return fail(ctx, CSE_ERR_RNG, "random state draw exceeded its shrink cap");
     if (errv & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
-
-    res->prep_ms = elapsed(ctx->e0, ctx->e1);
-    res->kernel_ms = elapsed(ctx->e1, ctx->e2);
-    res->ips = (int64_t)nips;
-    res->vanished = 0;
-    for (int64_t s = 0; s < count; s++)
-        if (se[s] < 0) res->vanished++;
-    cse_sampled_estimators(t, count, z, se, kz, &res->mean, &res->mean_kahan);
-    if (res->zsum) memcpy(res->zsum, z, 32 * n);
-    if (res->sexp) memcpy(res->sexp, se, 4 * n);
-    if (res->kahan) memcpy(res->kahan, kz, 16 * n);
+    const double pms = elapsed(ctx->e0, ctx->e1), kms = elapsed(ctx->e1, ctx->e2);
+    if (kernel_ms) *kernel_ms = kms;
+    auto convert = [&](int lo, int hi) {
+        for (int i = lo; i < hi; i++) {
+            cse_sampled_result *r = &res[i];
+            const size_t off = (size_t)pl.rec_off[i], cnt = (size_t)in[i].count;
+            r->prep_ms = pms;
+            r->kernel_ms = kms;
+            r->ips = (int64_t)nips[i];
+            r->vanished = 0;
+            for (size_t s = 0; s < cnt; s++)
+                if (se[off + s] < 0) r->vanished++;
+            cse_sampled_estimators(in[i].P.nqubits, (int64_t)cnt, z + 4 * off, se + off, kz + 2 * off, &r->mean,
+                                   &r->mean_kahan);
+            if (r->zsum) memcpy(r->zsum, z + 4 * off, 32 * cnt);
+            if (r->sexp) memcpy(r->sexp, se + off, 4 * cnt);
+            if (r->kahan) memcpy(r->kahan, kz + 2 * off, 16 * cnt);
+        }
+    };
+    // the estimators cost about as much per sample as the copies: split large
+    // batches of instances over host threads
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nthr = (ninst >= 8 && pl.nrec >= (1 << 16)) ? (int)std::min(8u, hw ? hw : 1u) : 1;
+    if (nthr > 1) {
+        std::vector<std::thread> pool;
+        const int per = (ninst + nthr - 1) / nthr;
+        for (int w = 0; w < nthr; w++) {
+            const int lo = w * per, hi = std::min(ninst, lo + per);
+            if (lo < hi) pool.emplace_back(convert, lo, hi);
+        }
+        for (std::thread &th : pool) th.join();
+    } else {
+        convert(0, ninst);
+    }
     return CSE_OK;
+}
+
+int cse_sampled_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k, int exact,
+                          uint64_t seed, int64_t first, int64_t count, int chunk, cse_sampled_result *res) {
+    if (!ctx || !res) return CSE_ERR_ARG;
+    int rc = check_projector(ctx, P);
+    if (rc) return rc;
+    rc = check_decomp(ctx, P->nqubits, k, exact, L);
+    if (rc) return rc;
+    if (count < 1) return fail(ctx, CSE_ERR_ARG, "count=%lld", (long long)count);
+    if (count > kMaxSamples) return fail(ctx, CSE_ERR_ARG, "count=%lld above 2^24 per call", (long long)count);
+    HIPCHK(hipSetDevice(ctx->device));
+    cse_sampled_instance in;
+    memset(&in, 0, sizeof(in));
+    in.P = *P;
+    in.exact = exact;
+    in.k = exact ? 0 : k;
+    if (!exact)
+        for (int r = 0; r < k; r++) in.L[r] = L[r];
+    in.seed = seed;
+    in.first = first;
+    in.count = count;
+    return sampled_batch(ctx, &in, 1, chunk, res, nullptr);
+}
+
+int cse_batch_sampled(cse_ctx *ctx, const cse_sampled_instance *inst, int ninst, int chunk,
+                      cse_sampled_result *res, double *kernel_ms) {
+    if (!ctx || !inst || !res || ninst < 1) return CSE_ERR_ARG;
+    long long total = 0;
+    for (int i = 0; i < ninst; i++) {
+        int rc = check_projector(ctx, &inst[i].P);
+        if (!rc) rc = check_decomp(ctx, inst[i].P.nqubits, inst[i].k, inst[i].exact, inst[i].L);
+        if (rc) return fail(ctx, rc, "instance %d: %s", i, ctx->err);
+        if (inst[i].count < 1) return fail(ctx, CSE_ERR_ARG, "instance %d: count=%lld", i, (long long)inst[i].count);
+        total += inst[i].count;
+        if (total > kMaxSamples) return fail(ctx, CSE_ERR_ARG, "more than 2^24 samples in one call");
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    return sampled_batch(ctx, inst, ninst, chunk, res, kernel_ms);
 }
 
 static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chunk, cse_exact_result *res,
@@ -555,7 +625,7 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
     if (!rc) rc = upload_plan(ctx, pl);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
-    rc = run_records(ctx, pl, 1, 0, 0);
+    rc = run_records(ctx, pl, 1);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e1, ctx->stream));
     rc = run_sweep(ctx, pl, 1, terms_out != nullptr);

@@ -222,13 +222,11 @@ void cse_sampled_estimators(int t, int64_t count, const long long *z, const int3
         Y = cse_dd_add(Y, dd_split_sum(bhi[se], blo[se], e));
     }
     cse_dd tot = cse_dd_add(X, cse_dd_mul(Y, SQRT2));
-    cse_dd inv = {1.0 / (double)count, 0};
     /* division by count: count is an integer < 2^53, refine once */
     double q = (tot.hi + tot.lo) / (double)count;
     cse_dd prod = cse_dd_mul((cse_dd){q, 0}, (cse_dd){(double)count, 0});
     cse_dd rem = cse_dd_add(tot, (cse_dd){-prod.hi, -prod.lo});
     q += (rem.hi + rem.lo) / (double)count;
-    (void)inv;
     *mean = q;
     *mean_kahan = kacc / (double)count;
 }

@@ -39,6 +39,8 @@ struct InstDesc {
     long long nchunks;  // work items per record
     long long rec_off;  // first record of this instance
     long long term_off; // first per-term output slot
+    uint64_t seed;      // mode 0: Philox key of the random states
+    long long first;    // mode 0: sample index of the first record
     uint64_t L[64];
 };
 

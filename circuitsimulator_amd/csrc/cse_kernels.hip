@@ -315,14 +315,14 @@ k_random_states(int n, const uint64_t *__restrict__ thr, uint64_t seed, long lon
 
 // Records of projected states, one wave per record g (global index over all
 // instances; instance found from the record prefix).
-// mode 0: theta_s = random stabilizer state s = first + local index
+// mode 0: theta_s = random stabilizer state s = first + local index of the
+//         instance's Philox stream (seed)
 // mode 1: phi_i = prepH(i) (exact) or prepL(i) (|L>), i = local index
 // then Pi (innerprod.c:95-116 / :218-241) and the elimination-ready record.
 extern "C" __global__ void __launch_bounds__(256)
 k_records(int mode, int ninst, const long long *__restrict__ rec_off, const InstDesc *__restrict__ inst,
-          const DevProjector *__restrict__ projs, const uint64_t *__restrict__ thr_table, uint64_t seed,
-          long long first, long long nrec, uint64_t *__restrict__ recs, int32_t *__restrict__ sexp_out,
-          int *__restrict__ err) {
+          const DevProjector *__restrict__ projs, const uint64_t *__restrict__ thr_table, long long nrec,
+          uint64_t *__restrict__ recs, int32_t *__restrict__ sexp_out, int *__restrict__ err) {
     const long long g = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (g >= nrec) return;
     int in = 0;
@@ -341,7 +341,7 @@ k_records(int mode, int ninst, const long long *__restrict__ rec_off, const Inst
     uint64_t *rec = recs + (size_t)REC_WORDS * (size_t)g;
     RState s;
     if (mode == 0) {
-        if (!rs_random_state(s, n, thr_table + 65 * n, seed, (uint64_t)(first + local))) {
+        if (!rs_random_state(s, n, thr_table + 65 * n, D.seed, (uint64_t)(D.first + local))) {
             if (lane_id() == 0) {
                 atomicOr(err, 1);
                 sexp_out[g] = -1;

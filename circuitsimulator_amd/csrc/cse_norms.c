@@ -30,22 +30,68 @@ static double clifford_sum(const cse_projector *P)
     return sum;
 }
 
-/* mean over samples [first, first + count), in launches of at most 2^20 */
+/* samples per engine call (about 1 GiB of device records) */
+#define CSE_CALL_SAMPLES (1 << 20)
+
+/* mean over samples [first, first + count), in calls of at most
+ * CSE_CALL_SAMPLES; a range that fits one call is that call's mean */
 static int sampled_mean(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k, int exact,
                         int64_t count, uint64_t seed, int64_t first, double *out)
 {
     double total = 0;
-    const int64_t step = 1 << 20;
+    const int64_t step = CSE_CALL_SAMPLES;
     for (int64_t s0 = 0; s0 < count; s0 += step) {
         const int64_t cnt = count - s0 < step ? count - s0 : step;
         cse_sampled_result r;
         memset(&r, 0, sizeof(r));
         const int rc = cse_sampled_projector(ctx, P, L, k, exact, seed, first + s0, cnt, 0, &r);
         if (rc != CSE_OK) return rc;
+        if (cnt == count) {
+            *out = r.mean;
+            return CSE_OK;
+        }
         total += r.mean * (double)cnt;
     }
     *out = total / (double)count;
     return CSE_OK;
+}
+
+/* vals[b] = mean over bin b's samples [b * samples, (b + 1) * samples): bins
+ * that fit one call together go through cse_batch_sampled, a bin per
+ * instance; larger bins are taken one at a time */
+static int bin_means(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k, int exact, int samples,
+                     int bins, uint64_t seed, double *vals)
+{
+    if (samples > CSE_CALL_SAMPLES / 2) {
+        for (int b = 0; b < bins; b++) {
+            const int rc = sampled_mean(ctx, P, L, k, exact, samples, seed, (int64_t)b * samples, &vals[b]);
+            if (rc != CSE_OK) return rc;
+        }
+        return CSE_OK;
+    }
+    const int per = CSE_CALL_SAMPLES / samples;
+    const int n = bins < per ? bins : per;
+    cse_sampled_instance *in = (cse_sampled_instance *)calloc((size_t)n, sizeof(cse_sampled_instance));
+    cse_sampled_result *res = (cse_sampled_result *)calloc((size_t)n, sizeof(cse_sampled_result));
+    int rc = (in && res) ? CSE_OK : CSE_ERR_NOMEM;
+    for (int i = 0; i < n && rc == CSE_OK; i++) {
+        in[i].P = *P;
+        in[i].exact = exact;
+        in[i].k = exact ? 0 : k;
+        for (int r = 0; !exact && r < k; r++) in[i].L[r] = L[r];
+        in[i].seed = seed;
+        in[i].count = samples;
+    }
+    for (int b0 = 0; b0 < bins && rc == CSE_OK; b0 += n) {
+        const int m = bins - b0 < n ? bins - b0 : n;
+        for (int i = 0; i < m; i++) in[i].first = (int64_t)(b0 + i) * samples;
+        memset(res, 0, sizeof(cse_sampled_result) * (size_t)m);
+        rc = cse_batch_sampled(ctx, in, m, 0, res, NULL);
+        for (int i = 0; rc == CSE_OK && i < m; i++) vals[b0 + i] = res[i].mean;
+    }
+    free(in);
+    free(res);
+    return rc;
 }
 
 static int cmp_double(const void *a, const void *b)
@@ -90,11 +136,9 @@ int cse_projector_norm(cse_ctx **ctx, int device, const cse_projector *P, const 
         if (rc == CSE_OK) *value = r.value;
         return rc;
     }
-    if (bins == 1) return sampled_mean(*ctx, P, L, k, exact, samples, seed, 0, value);
     double *vals = (double *)malloc(sizeof(double) * (size_t)bins);
     if (!vals) return CSE_ERR_NOMEM;
-    for (int b = 0; b < bins && rc == CSE_OK; b++)
-        rc = sampled_mean(*ctx, P, L, k, exact, samples, seed, (int64_t)b * samples, &vals[b]);
+    rc = bin_means(*ctx, P, L, k, exact, samples, bins, seed, vals);
     if (rc == CSE_OK) {
         qsort(vals, (size_t)bins, sizeof(double), cmp_double);
         *value = (bins % 2) ? vals[(bins - 1) / 2] : (vals[bins / 2] + vals[bins / 2 - 1]) / 2;

@@ -24,7 +24,7 @@ ABI_SYMBOLS = [
     "cse_create", "cse_destroy", "cse_error", "cse_device_info", "cse_inner_products",
     "cse_prep_states", "cse_random_states", "cse_exact_projector", "cse_sampled_projector",
     "cse_batch_exact", "cse_decompose", "cse_hist_value", "cse_sampling_kept", "cse_projector_norm",
-    "cse_lspace_weights",
+    "cse_lspace_weights", "cse_batch_sampled",
 ]
 ERR_NODEV = -5
 
@@ -76,6 +76,19 @@ def instance_array(n):
     return np.zeros(n, dtype=INSTANCE_DTYPE)
 
 
+class SampledInstance(ctypes.Structure):
+    _fields_ = [("P", Projector), ("exact", ctypes.c_int), ("k", ctypes.c_int), ("L", U64 * NMAX),
+                ("seed", U64), ("first", ctypes.c_int64), ("count", ctypes.c_int64)]
+
+
+SAMPLED_INSTANCE_DTYPE = np.dtype(SampledInstance)
+
+
+def sampled_instance_array(n):
+    """n zeroed struct cse_sampled_instance records as a numpy array."""
+    return np.zeros(n, dtype=SAMPLED_INSTANCE_DTYPE)
+
+
 _lib = None
 
 
@@ -102,6 +115,8 @@ def lib():
                                             U64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, P(SampledResult)]
         L.cse_batch_exact.argtypes = [ctypes.c_void_p, P(Instance), ctypes.c_int, ctypes.c_int, P(ExactResult),
                                       P(ctypes.c_int32), P(ctypes.c_double)]
+        L.cse_batch_sampled.argtypes = [ctypes.c_void_p, P(SampledInstance), ctypes.c_int, ctypes.c_int,
+                                        P(SampledResult), P(ctypes.c_double)]
         L.cse_decompose.argtypes = [ctypes.c_int, U64, P(U64), P(ctypes.c_double), P(ctypes.c_int),
                                     P(ctypes.c_int), ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int]
@@ -239,6 +254,27 @@ class Engine:
                                             _ptr(terms, ctypes.c_int32) if want_terms else None,
                                             ctypes.byref(ms)))
         return list(res), terms, ms.value
+
+    def batch_sampled(self, instances, chunk=0, want_samples=False):
+        """instances: a contiguous sampled_instance_array -> (means, results,
+        kernel ms, samples); means[i] is cse_sampled_projector's mean for
+        instance i; samples[i] (want_samples) its per-sample zsum and sexp."""
+        if instances.dtype != SAMPLED_INSTANCE_DTYPE or not instances.flags["C_CONTIGUOUS"]:
+            raise ValueError("instances must be a contiguous sampled_instance_array")
+        n = len(instances)
+        res = (SampledResult * n)()
+        samples = []
+        if want_samples:
+            for i in range(n):
+                cnt = int(instances["count"][i])
+                extra = {"zsum": np.zeros((cnt, 4), dtype=np.int64), "sexp": np.zeros(cnt, dtype=np.int32)}
+                res[i].zsum = _ptr(extra["zsum"], ctypes.c_int64)
+                res[i].sexp = _ptr(extra["sexp"], ctypes.c_int32)
+                samples.append(extra)
+        ms = ctypes.c_double()
+        self._check(self._L.cse_batch_sampled(self.h, instances.ctypes.data_as(P(SampledInstance)), n, chunk,
+                                              res, ctypes.byref(ms)))
+        return np.array([r.mean for r in res]), list(res), ms.value, samples
 
 
 def decompose(t, seed, exact=0, k=0, fidbound=1e-5, rank=0, fidelity=0, forceL=0, verbose=0, quiet=1):

@@ -152,6 +152,44 @@ def _bits(P, t):
             [np.array([(z >> j) & 1 for j in range(t)]) for z in zm])
 
 
+# samples per engine call in sampled mode, as cse_projector_norm batches them
+_CALL_SAMPLES = 1 << 20
+# the denominator's seed is the numerator's XOR this (projector_norms)
+_H_SEED = 0x9E3779B97F4A7C15
+
+
+def _median(vals):
+    """Median of each row as multiSampledProjector takes it
+    (innerprod.c:30-38): the middle value, or the mean of the middle two."""
+    s = np.sort(vals, axis=1)
+    b = s.shape[1]
+    return s[:, (b - 1) // 2] if b % 2 else (s[:, b // 2] + s[:, b // 2 - 1]) / 2
+
+
+def _sampled_medians(engine, phases, xs, zs, ex, k, Ls, seeds, samples, bins, fill, seed_xor):
+    """multiSampledProjector (innerprod.c:23-41) for every seed in seeds:
+    the projector (xs, zs) with phases one row per seed or one row for all,
+    the decomposition exact (ex) or row i of Ls.  Bin b of seed sd draws the
+    samples [b samples, (b + 1) samples) keyed by sd ^ seed_xor, as
+    cse_projector_norm does, so each median equals projector_norm's; every
+    bin of every seed is one instance of cse_batch_sampled, as many per call
+    as _CALL_SAMPLES allows."""
+    from .engine import sampled_instance_array
+    ns = len(seeds)
+    A = sampled_instance_array(ns * bins)
+    phases = np.asarray(phases)
+    rows = np.repeat(phases, bins, axis=0) if len(phases) == ns else phases
+    fill(A, slice(None), rows, xs, zs, ex, k, None if Ls is None else np.repeat(Ls, bins, axis=0))
+    A["seed"] = np.repeat(np.array(seeds, dtype=np.uint64) ^ np.uint64(seed_xor), bins)
+    A["first"] = np.tile(np.arange(bins, dtype=np.int64) * samples, ns)
+    A["count"] = samples
+    means = np.empty(ns * bins)
+    per = _CALL_SAMPLES // samples
+    for i0 in range(0, len(A), per):
+        means[i0:i0 + per] = engine.batch_sampled(A[i0:i0 + per])[0]
+    return _median(means.reshape(ns, bins))
+
+
 def sampleShots(circ, measure, sample, shots, config={}):
     """shots independent sampleQubits(circ, measure, sample, config) draws
     (libcirc/sample.py:16-86), advanced together one sampled qubit per
@@ -170,7 +208,10 @@ def sampleShots(circ, measure, sample, shots, config={}):
     reference's rule evaluates them exactly (probability.c:162-174; with the
     default 2000 samples this is every |L> of k <= 11 and every exact |H> of
     t <= 22); with the exact decomposition every distinct G' and H' is
-    evaluated once.  Otherwise each norm is sampled on its own.
+    evaluated once.  Otherwise the norms are sampled, every bin of every
+    shot's numerator and denominator an instance of one batched launch
+    (cse_batch_sampled, up to 2^20 samples per launch), each median equal
+    to what probability() computes with the same seed.
 
     Differences from calling sampleQubits shots times, none of which changes
     a probability: one T-ancilla postselection y serves the whole run (any y
@@ -252,13 +293,25 @@ def sampleShots(circ, measure, sample, shots, config={}):
         callcfg["exact"] = bool(exact) or len(Hm[0]) == 0
         seeds = [int(sd) for sd in rand_int(2 ** 62, len(todo))]
         Ls, norm, ex, k = decompose_many(t, callcfg, seeds)
-        if t == 0 or sampling_kept(callcfg.get("noapprox"), callcfg.get("forceSample"), ex, t, k,
-                                   int(callcfg.get("samples") or 0), int(callcfg.get("bins") or 0)):
-            # sampled norms, or the t = 0 closed form: one call each
-            for c, sd in zip(todo, seeds):
-                callcfg["seed"] = sd
-                num, den = projector_norms(_bits((list(PH[c]), xk, zk), t), _bits(Hm, t), t, callcfg)
-                out[c] = 0.0 if num == 0 else base * num / den
+        samples, bins = int(callcfg.get("samples") or 0), int(callcfg.get("bins") or 0)
+        if t == 0 or sampling_kept(callcfg.get("noapprox"), callcfg.get("forceSample"), ex, t, k, samples, bins):
+            if t == 0 or samples <= 0 or bins <= 0 or samples > _CALL_SAMPLES // 2:
+                # the t = 0 closed form, the degenerate settings and bins too
+                # large to share a call: one call each
+                for c, sd in zip(todo, seeds):
+                    callcfg["seed"] = sd
+                    num, den = projector_norms(_bits((list(PH[c]), xk, zk), t), _bits(Hm, t), t, callcfg)
+                    out[c] = 0.0 if num == 0 else base * num / den
+                return out
+            if engine is None:
+                engine = Engine(device)
+            num = _sampled_medians(engine, PH[todo], xk, zk, ex, k, Ls, seeds, samples, bins, fill, 0)
+            if Hm[0]:
+                den = _sampled_medians(engine, np.asarray([Hm[0]]), Hm[1], Hm[2], ex, k, Ls, seeds, samples,
+                                       bins, fill, _H_SEED)
+            else:
+                den = np.full(len(todo), norm * norm)
+            out[todo] = np.where(num == 0, 0.0, base * num / np.where(num == 0, 1.0, den))
             return out
         if ex:
             # no randomness: one record per distinct G', one for H'

@@ -160,6 +160,27 @@ typedef struct {
 int cse_batch_exact(cse_ctx *ctx, const cse_instance *inst, int ninst, int chunk,
                     cse_exact_result *res, int32_t *terms_out, double *kernel_ms);
 
+typedef struct {
+    cse_projector P;
+    int exact;          /* as in cse_sampled_projector */
+    int k;
+    uint64_t L[CSE_NMAX];
+    uint64_t seed;      /* Philox key of the random states */
+    int64_t first;      /* sample range [first, first + count) */
+    int64_t count;
+} cse_sampled_instance;
+
+/*
+ * Many sampled-mode instances (the bins of multiSampledProjector,
+ * innerprod.c:23-41, or the norms of many probability calls) in one record
+ * launch and one persistent sweep.  res[i] is exactly what
+ * cse_sampled_projector returns for instance i (its optional per-sample
+ * buffers filled when non-NULL; the timings are those of the whole call).
+ * At most 2^24 samples over all instances.
+ */
+int cse_batch_sampled(cse_ctx *ctx, const cse_sampled_instance *inst, int ninst, int chunk,
+                      cse_sampled_result *res, double *kernel_ms);
+
 /* --------------------------------------------------- host-side helpers */
 /*
  * decompose (libcirc/probability.c:307-415): norm of the decomposition, the

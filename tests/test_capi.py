@@ -39,6 +39,32 @@ def test_no_gpu_create_fails_cleanly():
         Engine(0)
 
 
+def test_struct_layouts_match_the_header(tmp_path):
+    # the ctypes mirrors (and the numpy record arrays built from them) agree
+    # with the C compiler's layout of every struct passed by pointer
+    from circuitsimulator_amd import engine as E
+    structs = {"cse_projector": E.Projector, "cse_exact_result": E.ExactResult,
+               "cse_sampled_result": E.SampledResult, "cse_instance": E.Instance,
+               "cse_sampled_instance": E.SampledInstance}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cse.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname, fname))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(ln.split() for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                   check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[cname + "." + fname]) == getattr(py, fname).offset, (cname, fname)
+    assert E.SAMPLED_INSTANCE_DTYPE.itemsize == ctypes.sizeof(E.SampledInstance)
+
+
 def test_product_does_not_reference_the_oracle():
     # the oracle is test infrastructure; the product never names it
     pkg = os.path.join(ROOT, "circuitsimulator_amd")

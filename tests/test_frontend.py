@@ -534,3 +534,37 @@ def test_batched_shots_constraints_and_L_decomposition():
     assert all(len(o) == 2 and set(o) <= {"0", "1"} for o in out)
     imp = sampleShots("X_\nT_\n", {0: 0}, [1], 10, config={"exact": True, "rng": np.random.default_rng(7)})
     assert imp == [""] * 10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exact,bins", [(False, 3), (True, 2)])
+def test_batched_shots_sampled_norms_match_per_call(monkeypatch, exact, bins):
+    # with sampling kept, every bin of every shot's numerator and denominator
+    # is an instance of cse_batch_sampled; the probabilities equal those of
+    # one projector_norms call per shot (forced by a per-call sample limit
+    # below the bin size) exactly
+    import circuitsimulator_amd.sample as S
+    circ = "H__\nT__\nCX_\nH__\nT__\n_CX\n__T\n__H\n"
+    cfg = {"exact": exact, "k": 2, "samples": 150, "bins": bins, "forceSample": True}
+
+    def run(call_samples):
+        seen = []
+        orig = S._next_psofar
+
+        def record(chain, P, P0, Psofar, bit):
+            seen.append(np.array(P))
+            return orig(chain, P, P0, Psofar, bit)
+
+        monkeypatch.setattr(S, "_next_psofar", record)
+        monkeypatch.setattr(S, "_CALL_SAMPLES", call_samples)
+        out = S.sampleShots(circ, {1: 0}, [0, 2], 12, config=dict(cfg, rng=np.random.default_rng(21)))
+        monkeypatch.setattr(S, "_next_psofar", orig)
+        return out, seen
+
+    out_b, P_b = run(1 << 20)
+    out_c, P_c = run(2)
+    assert out_b == out_c
+    assert len(P_b) == len(P_c) == 2
+    for a, b in zip(P_b, P_c):
+        assert np.array_equal(a, b)
+    assert all(np.isfinite(p).all() for p in P_b)

@@ -295,6 +295,77 @@ def test_sampled_chunks_across_batches(E, chunk):
     assert np.array_equal(ex["zsum"], zsum)
 
 
+@pytest.mark.parametrize("wide", [False, True])
+def test_batch_sampled_against_oracle_and_single_calls(E, wide):
+    # uneven sampled-mode instances (t, r, k, exact, seed, first, count) in
+    # one call: each instance's samples match the oracle and its mean equals
+    # cse_sampled_projector's bit for bit.  With a t = 61 instance the whole
+    # batch runs the unpacked sweep.
+    from circuitsimulator_amd.engine import (SAMPLED_INSTANCE_DTYPE, Projector, SampledInstance)
+    specs = [(6, 3, 3, 0, 5, 0, 7), (7, 2, 0, 1, 9, 100, 5), (10, 5, 5, 0, 5, 7, 1), (30, 8, 8, 0, 11, 2, 4),
+             (12, 6, 0, 1, 3, 40, 6), (24, 6, 7, 0, 5, 0, 9)]
+    if wide:
+        specs.insert(2, (61, 12, 3, 0, 4, 0, 2))
+    rng = np.random.default_rng(808 + wide)
+    arr = (SampledInstance * len(specs))()
+    data = []
+    for i, (t, r, k, exact, seed, first, count) in enumerate(specs):
+        ph, xs, zs = random_commuting_projector(rng, t, r)
+        L = [orc.rand_bits(rng, t) for _ in range(k)]
+        arr[i].P = Projector.make(ph, xs, zs, t)
+        arr[i].exact, arr[i].k = exact, k
+        for j, v in enumerate(L):
+            arr[i].L[j] = v
+        arr[i].seed, arr[i].first, arr[i].count = seed, first, count
+        data.append((ph, xs, zs, L))
+    A = np.frombuffer(arr, dtype=SAMPLED_INSTANCE_DTYPE)
+    means, res, _, samp = E.batch_sampled(A, want_samples=True)
+    for i, (t, r, k, exact, seed, first, count) in enumerate(specs):
+        ph, xs, zs, L = data[i]
+        chi = (1 << ((t + 1) // 2)) if exact else (1 << k)
+        se, zsum, _ = oracle_sampled(t, k, L, exact, ph, xs, zs, seed, first, count, chi)
+        assert np.array_equal(samp[i]["sexp"], se), i
+        assert np.array_equal(samp[i]["zsum"], zsum), i
+        one, _ = E.sampled_projector(arr[i].P, L, k, exact, seed, first, count, want_samples=False)
+        assert means[i] == one.mean, i
+        assert res[i].ips == one.ips == int((se >= 0).sum()) * chi
+        assert res[i].vanished == one.vanished == int((se < 0).sum())
+
+
+def test_batch_sampled_rejects_bad_counts(E):
+    # checked on the host before anything is uploaded or launched
+    from circuitsimulator_amd.engine import EngineError, sampled_instance_array
+    A = sampled_instance_array(2)
+    A["P"]["nqubits"] = 8
+    A["P"]["nstabs"] = 1
+    A["P"]["xs"][:, 0] = 1
+    A["k"] = 2
+    A["L"][:, :2] = [3, 12]
+    A["count"] = [5, 0]
+    with pytest.raises(EngineError, match="count=0"):
+        E.batch_sampled(A)
+    A["count"] = [1 << 23, (1 << 23) + 1]
+    with pytest.raises(EngineError, match="2\\^24"):
+        E.batch_sampled(A)
+
+
+@pytest.mark.parametrize("bins", [1, 3, 4])
+def test_projector_norm_bins_in_one_call(E, bins):
+    # cse_projector_norm sends every bin as one instance of one batched call;
+    # the median equals that of one cse_sampled_projector call per bin
+    from circuitsimulator_amd.engine import Projector, projector_norm
+    t, r, k, samples, seed = 16, 6, 5, 120, 4242
+    rng = np.random.default_rng(1600 + bins)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    L = [orc.rand_bits(rng, t) for _ in range(k)]
+    P = Projector.make(ph, xs, zs, t)
+    got = projector_norm(P, L, k, 0, 1.0, 1, samples, bins, seed)
+    vals = sorted(E.sampled_projector(P, L, k, 0, seed, b * samples, samples, want_samples=False)[0].mean
+                  for b in range(bins))
+    want = vals[(bins - 1) // 2] if bins % 2 else (vals[bins // 2] + vals[bins // 2 - 1]) / 2
+    assert got == want
+
+
 @pytest.mark.parametrize("t", [60, 61])
 def test_sampled_layout_limits(E, t):
     # n = 60 is the packed layout's limit (row 60 is the loop's end row);
