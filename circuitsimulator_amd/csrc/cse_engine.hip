@@ -153,15 +153,31 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
     return (double)ms;
 }
 
-int default_chunk(long long chi) {
-    long long c = 256;
+int default_chunk(long long chi, int size = 256) {
+    long long c = size;
     if (chi < c) c = chi;
     return (int)(c < 1 ? 1 : c);
 }
 
-// Persistent grid for k_sweep
-int sweep_grid(cse_ctx *ctx, long long nitems) {
-    long long g = (long long)ctx->cus * ctx->blocks_per_cu;
+// Exact mode's default item size: 512 j when the launch still has eight
+// items per resident wave of k_sweep_packed_all at that size (seven waves per
+// SIMD), else 256.  Larger items halve the per-item record load, table set-up
+// and histogram flush; smaller ones keep every wave busy on small launches.
+int exact_chunk(cse_ctx *ctx, const cse_instance *in, int ninst) {
+    long long items = 0;
+    for (int i = 0; i < ninst; i++) {
+        const long long chi = in[i].exact ? (1ll << ((in[i].P.nqubits + 1) / 2)) : (1ll << in[i].k);
+        // items of record r that reach the diagonal or beyond
+        const long long per = (chi + 511) / 512;
+        items += chi * (per + 1) / 2;
+    }
+    const long long waves = (long long)ctx->cus * 7 * 4;
+    return items >= 8 * waves ? 512 : 256;
+}
+
+// Persistent grid for k_sweep: bpc resident blocks of 4 waves per CU
+int sweep_grid(cse_ctx *ctx, long long nitems, int bpc) {
+    long long g = (long long)ctx->cus * bpc;
     long long need = (nitems + 3) / 4;
     if (need < g) g = need;
     return (int)(g < 1 ? 1 : g);
@@ -259,10 +275,12 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     A.hist_stride = HIST_STRIDE;
     A.ips = dp<unsigned long long>(ctx->ips);
     A.err = dp<int>(ctx->errflag);
-    const int grid = sweep_grid(ctx, pl.nitems);
     bool packed = true;
     for (const InstDesc &d : pl.inst) packed = packed && d.n <= CSE_END_ROW;
-    if (packed) hipLaunchKernelGGL(k_sweep_packed, dim3(grid), dim3(256), 0, ctx->stream, A);
+    // k_sweep_packed_all holds seven waves per SIMD, the others eight
+    const int grid = sweep_grid(ctx, pl.nitems, packed && mode == 1 ? 7 : ctx->blocks_per_cu);
+    if (packed && mode == 1) hipLaunchKernelGGL(k_sweep_packed_all, dim3(grid), dim3(256), 0, ctx->stream, A);
+    else if (packed) hipLaunchKernelGGL(k_sweep_packed, dim3(grid), dim3(256), 0, ctx->stream, A);
     else hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(256), 0, ctx->stream, A);
     HIPCHK(hipGetLastError());
     return CSE_OK;
@@ -275,7 +293,7 @@ InstDesc make_inst(int t, int k, int exact, const uint64_t *L, int chunk) {
     d.k = exact ? 0 : k;
     d.exact = exact ? 1 : 0;
     d.chi = exact ? (1ll << ((t + 1) / 2)) : (1ll << k);
-    d.chunk = chunk > 0 ? chunk : default_chunk(d.chi);
+    d.chunk = default_chunk(d.chi, chunk > 0 ? chunk : 256);
     if (d.chunk > d.chi) d.chunk = (int)d.chi;
     d.nchunks = (d.chi + d.chunk - 1) / d.chunk;
     if (!exact)
@@ -605,6 +623,7 @@ int cse_batch_sampled(cse_ctx *ctx, const cse_sampled_instance *inst, int ninst,
 
 static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chunk, cse_exact_result *res,
                        int32_t *terms_out, double *kernel_ms) {
+    if (chunk <= 0) chunk = exact_chunk(ctx, in, ninst);
     Plan pl;
     pl.rec_off.push_back(0);
     pl.item_off.push_back(0);

@@ -15,6 +15,8 @@
 //   k_sweep_packed   the same with D packed into the J rows (n <= 60); the
 //                    |L> terms of records with at most two non-pivot columns
 //                    run 64 at a time in one asm block (phiL_batch)
+//   k_sweep_packed_all  the packed sweep of exact mode: every |L> term in
+//                    the asm block, any number of restrictions
 //   k_lspace_hist    histogram of |x| over the row space of L (decompose's
 //                    fidelity)
 #include "cse_device.h"
@@ -422,7 +424,7 @@ DEV u64 lrow_combo(const uint64_t *L, int k, u64 j) {
 // (col0 for the lowest, col1 for the other) and h bits (hq bit 0, bit 1),
 // Q, and 16 k + 1 with k the record's dimension.
 struct PhiLConst {
-    u64 actR, actH, D1, D2, nonact, col0, col1;
+    u64 actR, actH, D1, D2, nonact, col0, col1, h;
     unsigned hq, Q, K16;
 };
 
@@ -459,142 +461,235 @@ DEV int phiL_unpack(int w) {
 //    S = col_q & act the constraint S . z = h_q + |col_q & B| mod 2 is an
 //    empty sum (S = 0, right side 1), nothing (S = 0, right side 0), or an
 //    elimination of min(S) as a constrained step whose variable leaves the
-//    exponent (W += 16) and adds no factor; a second column then
-//    sees G_b ^= G_i on rows b in S, i.e. col1 ^= S and h_q1 ^= c when
-//    G_iq1 = 1;
+//    exponent (W += 16) and adds no factor; later columns then see
+//    G_b ^= G_i on the rows b in S and h ^= G_i when c = 1;
 //  * the elimination loop from W = table + 4 |(E ^ D2) & B| in m, whose low
 //    field ends as the packed exponent; 0 for a vanishing sum.
+// ALL = false (records with at most two non-pivot columns): the columns are
+// col0 (the lowest) and col1 with their h bits in hq, the second one updated
+// in scalar registers (col1 ^= S and h_q1 ^= c when G_iq1 = 1).
+// ALL = true (any number): every column is a ballot of bit q of working G
+// rows (glo, ghi; reset from the record's rows g0 at a term's first
+// restriction), its h bit from a working h (s[48:49], reset from h0).
 // Fixed registers besides the loop's: m0 the term index (the lane the
 // tables are read from and v_writelane writes; s61 keeps the caller's m0),
-// s[46:47] x_j, s[44:45] B, s[48:49] and s50 the column and h bit of the
-// restriction at hand, s51 the constraint's right side / a temporary,
-// s[52:53] the edge ballot, s54 the table's W, s[62:63] the pending
-// non-pivot columns before the loop.
+// s[46:47] x_j (then G_i), s[44:45] B, s[48:49] and s50 the column and h
+// bit of the restriction at hand (ALL: s[48:49] the working h), s51 the
+// constraint's right side / a temporary, s[52:53] the edge ballot / the
+// column, s54 the table's W, s[62:63] the pending non-pivot columns before
+// the loop.
+#define PHIL_TERM \
+    "v_mov_b32 %[parked], 0\n\t" \
+    "v_mov_b32 v40, 0xc0000000\n\t" \
+    "s_mov_b32 s61, m0\n\t" \
+    "s_mov_b32 m0, 0\n\t" \
+    "s_nop 0\n" \
+    "L_ip%=:\n\t" \
+    "v_readlane_b32 s46, %[xtlo], m0\n\t" \
+    "v_readlane_b32 s47, %[xthi], m0\n\t" \
+    "v_readlane_b32 s54, %[ttab], m0\n\t" \
+    /* fixation */ \
+    "s_andn2_b64 s[44:45], %[actH], s[46:47]\n\t" \
+    "s_and_b64 s[58:59], %[actR], s[46:47]\n\t" \
+    "v_and_b32 %[vt0], s44, %[jlo0]\n\t" \
+    "v_and_b32 %[vt1], s45, %[jhi0]\n\t" \
+    "v_and_b32 %[vt3], %[vt1], %[lxhi]\n\t" \
+    /* (S0 & S1) ^ S2: the parity word of J_x & B below x */ \
+    "v_bitop3_b32 %[vt2], %[vt0], %[lxlo], %[vt3] bitop3:0x6a\n\t" \
+    "v_xor_b32 %[vt3], %[vt0], %[vt1]\n\t" \
+    "v_bcnt_u32_b32 %[vt2], %[vt2], 0\n\t" \
+    "v_bcnt_u32_b32 %[vt3], %[vt3], 0\n\t" \
+    "v_and_b32 %[vt2], 1, %[vt2]\n\t" \
+    "v_cmp_ne_u32_e64 s[52:53], 0, %[vt2]\n\t" \
+    "v_mov_b32 %[jlo], %[jlo0]\n\t" \
+    "v_lshl_add_u32 %[jhi], %[vt3], 31, %[jhi0]\n\t" \
+    "s_xor_b64 s[52:53], s[52:53], %[D2]\n\t" \
+    "s_and_b64 s[52:53], s[52:53], s[44:45]\n\t" \
+    "s_bcnt1_i32_b64 s51, s[52:53]\n\t" \
+    "s_lshl_b32 s51, s51, 31\n\t" \
+    "s_add_u32 s60, s54, s51\n\t" \
+    "s_andn2_b64 s[62:63], %[nonact], s[46:47]\n\t" \
+    "s_cbranch_scc1 L_rs%=\n" \
+    "L_elim%=:\n" \
+    CSE_ELIM_LOOP \
+    "L_end%=:\n\t" \
+    "s_cmp_eq_u32 s56, 60\n\t" \
+    "s_cselect_b32 s60, s60, 0\n" \
+    "L_emit%=:\n\t" \
+    "v_writelane_b32 %[parked], s60, m0\n\t" \
+    "s_add_u32 m0, m0, 1\n\t" \
+    "s_cmp_lt_u32 m0, %[nb]\n\t" \
+    "s_cbranch_scc1 L_ip%=\n\t" \
+    "s_branch L_done%=\n" \
+    "L_zero%=:\n\t" \
+    "s_mov_b32 s60, 0\n\t" \
+    "s_branch L_emit%=\n"
+
+#define PHIL_RESTRICT2 \
+    /* restrictions: s[62:63] = the non-pivot columns outside supp(x_j) */ \
+    "L_rs%=:\n\t" \
+    "s_ff1_i32_b64 s51, %[nonact]\n\t" \
+    "s_bitcmp1_b64 s[62:63], s51\n\t" \
+    "s_cbranch_scc0 L_rs1%=\n\t" \
+    "s_mov_b64 s[48:49], %[col0]\n\t" \
+    "s_and_b32 s50, %[hq], 1\n\t" \
+    "s_bitset0_b64 s[62:63], s51\n\t" \
+    "s_branch L_rdo%=\n" \
+    "L_rs1%=:\n\t" \
+    "s_mov_b64 s[48:49], %[col1]\n\t" \
+    "s_lshr_b32 s50, %[hq], 1\n\t" \
+    "s_mov_b64 s[62:63], 0\n" \
+    "L_rdo%=:\n\t" \
+    "s_and_b64 s[64:65], s[48:49], s[58:59]\n\t" \
+    "s_and_b64 s[52:53], s[48:49], s[44:45]\n\t" \
+    "s_bcnt1_i32_b64 s51, s[52:53]\n\t" \
+    "s_xor_b32 s51, s51, s50\n\t" \
+    "s_cmp_lg_u64 s[64:65], 0\n\t" \
+    "s_cbranch_scc0 L_rempty%=\n\t" \
+    "s_ff1_i32_b64 s56, s[64:65]\n\t" \
+    "s_bitset0_b64 s[58:59], s56\n\t" \
+    "v_readlane_b32 s66, %[jlo], s56\n\t" \
+    "v_readlane_b32 s67, %[jhi], s56\n\t" \
+    /* the second column after this elimination */ \
+    "s_bitcmp1_b64 %[col1], s56\n\t" \
+    "s_cselect_b64 s[52:53], s[64:65], 0\n\t" \
+    "s_cselect_b32 s54, s51, 0\n\t" \
+    "s_xor_b64 s[48:49], %[col1], s[52:53]\n\t" \
+    "s_lshr_b32 s50, %[hq], 1\n\t" \
+    "s_xor_b32 s50, s50, s54\n\t" \
+    /* elimination of i over S with c = the constraint's right side */ \
+    "s_and_b32 s56, s67, 0xe0000000\n\t" \
+    "s_and_b32 s67, s67, 0x1fffffff\n\t" \
+    "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t" \
+    "s_nop 1\n\t" \
+    "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t" \
+    "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t" \
+    "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t" \
+    "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t" \
+    "s_bitcmp1_b32 s56, 30\n\t" \
+    "s_cbranch_scc0 L_rnojii%=\n\t" \
+    "v_xor_b32 %[vt3], %[vt3], %[vt0]\n" \
+    "L_rnojii%=:\n\t" \
+    "s_add_u32 s60, s60, 0x10\n\t" \
+    "s_bitcmp1_b32 s51, 0\n\t" \
+    "s_cbranch_scc0 L_rnoc%=\n\t" \
+    "s_or_b32 s65, s65, 0x80000000\n\t" \
+    "s_add_u32 s60, s60, s56\n" \
+    "L_rnoc%=:\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n" \
+    "L_rnext%=:\n\t" \
+    "s_cmp_lg_u64 s[62:63], 0\n\t" \
+    "s_cbranch_scc0 L_elim%=\n\t" \
+    "s_mov_b64 s[62:63], 0\n\t" \
+    "s_branch L_rdo%=\n" \
+    "L_rempty%=:\n\t" \
+    "s_bitcmp1_b32 s51, 0\n\t" \
+    "s_cbranch_scc1 L_zero%=\n\t" \
+    "s_mov_b64 s[48:49], %[col1]\n\t" \
+    "s_lshr_b32 s50, %[hq], 1\n\t" \
+    "s_branch L_rnext%=\n"
+
+#define PHIL_RESTRICT_ALL \
+    "L_rs%=:\n\t" \
+    "v_mov_b32 %[glo], %[g0lo]\n\t" \
+    "v_mov_b32 %[ghi], %[g0hi]\n\t" \
+    "s_mov_b64 s[48:49], %[h0]\n" \
+    "L_rdo%=:\n\t" \
+    "s_ff1_i32_b64 s51, s[62:63]\n\t" \
+    "s_bitset0_b64 s[62:63], s51\n\t" \
+    "s_lshl_b64 s[52:53], 1, s51\n\t" \
+    "s_lshr_b64 s[66:67], s[48:49], s51\n\t" \
+    /* the column q over the rows: a ballot of bit q of the G rows */ \
+    "v_and_b32 %[vt0], s52, %[glo]\n\t" \
+    "v_and_or_b32 %[vt0], s53, %[ghi], %[vt0]\n\t" \
+    "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t" \
+    "s_and_b64 s[64:65], s[52:53], s[58:59]\n\t" \
+    "s_and_b64 s[52:53], s[52:53], s[44:45]\n\t" \
+    "s_bcnt1_i32_b64 s51, s[52:53]\n\t" \
+    "s_xor_b32 s51, s51, s66\n\t" \
+    "s_cmp_lg_u64 s[64:65], 0\n\t" \
+    "s_cbranch_scc0 L_rempty%=\n\t" \
+    "s_ff1_i32_b64 s56, s[64:65]\n\t" \
+    "s_bitset0_b64 s[58:59], s56\n\t" \
+    "v_readlane_b32 s66, %[jlo], s56\n\t" \
+    "v_readlane_b32 s67, %[jhi], s56\n\t" \
+    "v_readlane_b32 s46, %[glo], s56\n\t" \
+    "v_readlane_b32 s47, %[ghi], s56\n\t" \
+    /* elimination of i over S with c = the constraint's right side */ \
+    "s_and_b32 s56, s67, 0xe0000000\n\t" \
+    "s_and_b32 s67, s67, 0x1fffffff\n\t" \
+    "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t" \
+    "s_nop 1\n\t" \
+    "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t" \
+    "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t" \
+    "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t" \
+    "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t" \
+    /* the G rows of S take G_i (row i becomes 0: it is retired) */ \
+    "v_bitop3_b32 %[glo], %[glo], s46, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[ghi], %[ghi], s47, %[vt0] bitop3:0x78\n\t" \
+    "s_bitcmp1_b32 s56, 30\n\t" \
+    "s_cbranch_scc0 L_rnojii%=\n\t" \
+    "v_xor_b32 %[vt3], %[vt3], %[vt0]\n" \
+    "L_rnojii%=:\n\t" \
+    "s_add_u32 s60, s60, 0x10\n\t" \
+    "s_bitcmp1_b32 s51, 0\n\t" \
+    "s_cbranch_scc0 L_rnoc%=\n\t" \
+    "s_or_b32 s65, s65, 0x80000000\n\t" \
+    "s_add_u32 s60, s60, s56\n\t" \
+    "s_xor_b64 s[48:49], s[48:49], s[46:47]\n" \
+    "L_rnoc%=:\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n" \
+    "L_rnext%=:\n\t" \
+    "s_cmp_lg_u64 s[62:63], 0\n\t" \
+    "s_cbranch_scc1 L_rdo%=\n\t" \
+    "s_branch L_elim%=\n" \
+    "L_rempty%=:\n\t" \
+    "s_bitcmp1_b32 s51, 0\n\t" \
+    "s_cbranch_scc1 L_zero%=\n\t" \
+    "s_branch L_rnext%=\n"
+
+template <bool ALL>
 DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, unsigned lxhi, unsigned xtlo,
-                   unsigned xthi, unsigned ttab, const PhiLConst &C) {
+                   unsigned xthi, unsigned ttab, const PhiLConst &C, u64 g0) {
     int parked;
     unsigned jlo, jhi, vt0, vt1, vt2, vt3;
-    asm volatile(
-        "v_mov_b32 %[parked], 0\n\t"
-        "v_mov_b32 v40, 0xc0000000\n\t"
-        "s_mov_b32 s61, m0\n\t"
-        "s_mov_b32 m0, 0\n\t"
-        "s_nop 0\n"
-        "L_ip%=:\n\t"
-        "v_readlane_b32 s46, %[xtlo], m0\n\t"
-        "v_readlane_b32 s47, %[xthi], m0\n\t"
-        "v_readlane_b32 s54, %[ttab], m0\n\t"
-        // fixation
-        "s_andn2_b64 s[44:45], %[actH], s[46:47]\n\t"
-        "s_and_b64 s[58:59], %[actR], s[46:47]\n\t"
-        "v_and_b32 %[vt0], s44, %[jlo0]\n\t"
-        "v_and_b32 %[vt1], s45, %[jhi0]\n\t"
-        "v_and_b32 %[vt3], %[vt1], %[lxhi]\n\t"
-        // (S0 & S1) ^ S2: the parity word of J_x & B below x
-        "v_bitop3_b32 %[vt2], %[vt0], %[lxlo], %[vt3] bitop3:0x6a\n\t"
-        "v_xor_b32 %[vt3], %[vt0], %[vt1]\n\t"
-        "v_bcnt_u32_b32 %[vt2], %[vt2], 0\n\t"
-        "v_bcnt_u32_b32 %[vt3], %[vt3], 0\n\t"
-        "v_and_b32 %[vt2], 1, %[vt2]\n\t"
-        "v_cmp_ne_u32_e64 s[52:53], 0, %[vt2]\n\t"
-        "v_mov_b32 %[jlo], %[jlo0]\n\t"
-        "v_lshl_add_u32 %[jhi], %[vt3], 31, %[jhi0]\n\t"
-        "s_xor_b64 s[52:53], s[52:53], %[D2]\n\t"
-        "s_and_b64 s[52:53], s[52:53], s[44:45]\n\t"
-        "s_bcnt1_i32_b64 s51, s[52:53]\n\t"
-        "s_lshl_b32 s51, s51, 31\n\t"
-        "s_add_u32 s60, s54, s51\n\t"
-        "s_andn2_b64 s[62:63], %[nonact], s[46:47]\n\t"
-        "s_cbranch_scc1 L_rs%=\n"
-        "L_elim%=:\n"
-        CSE_ELIM_LOOP
-        "L_end%=:\n\t"
-        "s_cmp_eq_u32 s56, 60\n\t"
-        "s_cselect_b32 s60, s60, 0\n"
-        "L_emit%=:\n\t"
-        "v_writelane_b32 %[parked], s60, m0\n\t"
-        "s_add_u32 m0, m0, 1\n\t"
-        "s_cmp_lt_u32 m0, %[nb]\n\t"
-        "s_cbranch_scc1 L_ip%=\n\t"
-        "s_branch L_done%=\n"
-        "L_zero%=:\n\t"
-        "s_mov_b32 s60, 0\n\t"
-        "s_branch L_emit%=\n"
-        // restrictions: s[62:63] = the non-pivot columns outside supp(x_j)
-        "L_rs%=:\n\t"
-        "s_ff1_i32_b64 s51, %[nonact]\n\t"
-        "s_bitcmp1_b64 s[62:63], s51\n\t"
-        "s_cbranch_scc0 L_rs1%=\n\t"
-        "s_mov_b64 s[48:49], %[col0]\n\t"
-        "s_and_b32 s50, %[hq], 1\n\t"
-        "s_bitset0_b64 s[62:63], s51\n\t"
-        "s_branch L_rdo%=\n"
-        "L_rs1%=:\n\t"
-        "s_mov_b64 s[48:49], %[col1]\n\t"
-        "s_lshr_b32 s50, %[hq], 1\n\t"
-        "s_mov_b64 s[62:63], 0\n"
-        "L_rdo%=:\n\t"
-        "s_and_b64 s[64:65], s[48:49], s[58:59]\n\t"
-        "s_and_b64 s[52:53], s[48:49], s[44:45]\n\t"
-        "s_bcnt1_i32_b64 s51, s[52:53]\n\t"
-        "s_xor_b32 s51, s51, s50\n\t"
-        "s_cmp_lg_u64 s[64:65], 0\n\t"
-        "s_cbranch_scc0 L_rempty%=\n\t"
-        "s_ff1_i32_b64 s56, s[64:65]\n\t"
-        "s_bitset0_b64 s[58:59], s56\n\t"
-        "v_readlane_b32 s66, %[jlo], s56\n\t"
-        "v_readlane_b32 s67, %[jhi], s56\n\t"
-        // the second column after this elimination
-        "s_bitcmp1_b64 %[col1], s56\n\t"
-        "s_cselect_b64 s[52:53], s[64:65], 0\n\t"
-        "s_cselect_b32 s54, s51, 0\n\t"
-        "s_xor_b64 s[48:49], %[col1], s[52:53]\n\t"
-        "s_lshr_b32 s50, %[hq], 1\n\t"
-        "s_xor_b32 s50, s50, s54\n\t"
-        // elimination of i over S with c = the constraint's right side
-        "s_and_b32 s56, s67, 0xe0000000\n\t"
-        "s_and_b32 s67, s67, 0x1fffffff\n\t"
-        "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t"
-        "s_nop 1\n\t"
-        "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"
-        "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"
-        "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t"
-        "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t"
-        "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t"
-        "s_bitcmp1_b32 s56, 30\n\t"
-        "s_cbranch_scc0 L_rnojii%=\n\t"
-        "v_xor_b32 %[vt3], %[vt3], %[vt0]\n"
-        "L_rnojii%=:\n\t"
-        "s_add_u32 s60, s60, 0x10\n\t"
-        "s_bitcmp1_b32 s51, 0\n\t"
-        "s_cbranch_scc0 L_rnoc%=\n\t"
-        "s_or_b32 s65, s65, 0x80000000\n\t"
-        "s_add_u32 s60, s60, s56\n"
-        "L_rnoc%=:\n\t"
-        "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n"
-        "L_rnext%=:\n\t"
-        "s_cmp_lg_u64 s[62:63], 0\n\t"
-        "s_cbranch_scc0 L_elim%=\n\t"
-        "s_mov_b64 s[62:63], 0\n\t"
-        "s_branch L_rdo%=\n"
-        "L_rempty%=:\n\t"
-        "s_bitcmp1_b32 s51, 0\n\t"
-        "s_cbranch_scc1 L_zero%=\n\t"
-        "s_mov_b64 s[48:49], %[col1]\n\t"
-        "s_lshr_b32 s50, %[hq], 1\n\t"
-        "s_branch L_rnext%=\n"
-        "L_done%=:\n\t"
-        "s_mov_b32 m0, s61\n\t"
-        "s_nop 1\n\t"
-        : [parked] "=&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
-          [vt2] "=&v"(vt2), [vt3] "=&v"(vt3)
-        : [nb] "s"(nb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [lxlo] "v"(lxlo), [lxhi] "v"(lxhi),
-          [xtlo] "v"(xtlo), [xthi] "v"(xthi), [ttab] "v"(ttab), [actR] "s"(C.actR), [actH] "s"(C.actH),
-          [D2] "s"(C.D2), [nonact] "s"(C.nonact), [col0] "s"(C.col0), [col1] "s"(C.col1), [hq] "s"(C.hq)
-        : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
-          "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+    if constexpr (!ALL) {
+        asm volatile(
+            PHIL_TERM
+            PHIL_RESTRICT2
+            "L_done%=:\n\t"
+            "s_mov_b32 m0, s61\n\t"
+            "s_nop 1\n\t"
+            : [parked] "=&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
+              [vt2] "=&v"(vt2), [vt3] "=&v"(vt3)
+            : [nb] "s"(nb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [lxlo] "v"(lxlo), [lxhi] "v"(lxhi),
+              [xtlo] "v"(xtlo), [xthi] "v"(xthi), [ttab] "v"(ttab), [actR] "s"(C.actR), [actH] "s"(C.actH),
+              [D2] "s"(C.D2), [nonact] "s"(C.nonact), [col0] "s"(C.col0), [col1] "s"(C.col1), [hq] "s"(C.hq)
+            : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
+              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+    } else {
+        unsigned glo, ghi;
+        asm volatile(
+            PHIL_TERM
+            PHIL_RESTRICT_ALL
+            "L_done%=:\n\t"
+            "s_mov_b32 m0, s61\n\t"
+            "s_nop 1\n\t"
+            : [parked] "=&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
+              [vt2] "=&v"(vt2), [vt3] "=&v"(vt3), [glo] "=&v"(glo), [ghi] "=&v"(ghi)
+            : [nb] "s"(nb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [lxlo] "v"(lxlo), [lxhi] "v"(lxhi),
+              [xtlo] "v"(xtlo), [xthi] "v"(xthi), [ttab] "v"(ttab), [actR] "s"(C.actR), [actH] "s"(C.actH),
+              [D2] "s"(C.D2), [nonact] "s"(C.nonact), [g0lo] "v"((unsigned)g0), [g0hi] "v"((unsigned)(g0 >> 32)),
+              [h0] "s"(C.h)
+            : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
+              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+    }
     return parked;
 }
 
@@ -747,7 +842,7 @@ DEV int fold_terms(int packed, int cnt, int mode, int diag, int sexp, LaneAcc &a
 // contribute to per-record Z[w] sums.  mode 1 (exact): record r is theta'_i
 // of its instance, j in [i, chi), weight 1 on i == j and 2 Re otherwise, into
 // the instance's (P, M) histogram.
-template <bool PK>
+template <bool PK, bool ALL>
 DEV void sweep_body(const SweepArgs &A) {
     __shared__ unsigned int lhist[4][LH_BINS];
     const int wv = (int)(threadIdx.x >> 6);
@@ -818,19 +913,23 @@ DEV void sweep_body(const SweepArgs &A) {
         } else {
             const int k = D.k;
             const u64 nonact = lowmask(n) & ~rv.act;
-            if (PK && __popcll(nonact) <= 2) {
+            if (PK && (ALL || __popcll(nonact) <= 2)) {
                 // the whole batch in one asm block
                 PhiLConst C;
-                const u64 q0m = nonact & (0ull - nonact), q1m = nonact & ~q0m;
-                const int q0 = q0m ? __builtin_ctzll(q0m) : 0, q1 = q1m ? __builtin_ctzll(q1m) : 0;
                 C.actR = rv.act | bit(CSE_END_ROW) | bit(63);
                 C.actH = rv.act & rv.h;
                 C.D1 = rv.D1;
                 C.D2 = rv.D2;
                 C.nonact = nonact;
-                C.col0 = q0m ? (ballot((rv.G >> q0) & 1ull) & rv.act) : 0ull;
-                C.col1 = q1m ? (ballot((rv.G >> q1) & 1ull) & rv.act) : 0ull;
-                C.hq = (unsigned)((q0m ? (rv.h >> q0) & 1ull : 0ull) | (q1m ? ((rv.h >> q1) & 1ull) << 1 : 0ull));
+                C.h = rv.h;
+                if (!ALL) {
+                    const u64 q0m = nonact & (0ull - nonact), q1m = nonact & ~q0m;
+                    const int q0 = q0m ? __builtin_ctzll(q0m) : 0, q1 = q1m ? __builtin_ctzll(q1m) : 0;
+                    C.col0 = q0m ? (ballot((rv.G >> q0) & 1ull) & rv.act) : 0ull;
+                    C.col1 = q1m ? (ballot((rv.G >> q1) & 1ull) & rv.act) : 0ull;
+                    C.hq = (unsigned)((q0m ? (rv.h >> q0) & 1ull : 0ull) |
+                                      (q1m ? ((rv.h >> q1) & 1ull) << 1 : 0ull));
+                }
                 C.Q = (unsigned)rv.Q;
                 C.K16 = 16u * (unsigned)rv.k + 1u;
                 const unsigned jlo0 = (unsigned)rv.J, jhi0 = (unsigned)(rv.J >> 32);
@@ -864,8 +963,9 @@ DEV void sweep_body(const SweepArgs &A) {
                     }
                     unsigned xtlo, xthi, ttab;
                     phiL_tables(C, xv, xtlo, xthi, ttab);
-                    const int parked = phiL_unpack(phiL_batch((unsigned)nb, jlo0, jhi0, (unsigned)lowx,
-                                                              (unsigned)(lowx >> 32), xtlo, xthi, ttab, C));
+                    const int parked = phiL_unpack(phiL_batch<ALL>((unsigned)nb, jlo0, jhi0, (unsigned)lowx,
+                                                                   (unsigned)(lowx >> 32), xtlo, xthi, ttab, C,
+                                                                   rv.G));
                     flush(parked, nb, jb);
                 }
             } else {
@@ -926,14 +1026,22 @@ DEV void sweep_body(const SweepArgs &A) {
     }
 }
 
-extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_body<false>(A); }
+extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_body<false, false>(A); }
 
 // the same sweep with D packed into the J rows (every instance has n <= 60;
 // row 60 is the elimination loop's end row).
-// Eight waves per SIMD (60 VGPRs, 78 SGPRs): the elimination loop is bound by
+// Eight waves per SIMD (63 VGPRs, 78 SGPRs): the elimination loop is bound by
 // instruction issue, and the extra wave hides more of its VALU -> SALU
 // dependency latency.
-extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed(SweepArgs A) { sweep_body<true>(A); }
+extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed(SweepArgs A) { sweep_body<true, false>(A); }
+
+// The packed sweep of exact mode: its records (projected |L> states of
+// dimension about |x_i|) have about n/2 non-pivot columns, so every |L> term
+// runs in the asm block with its general restriction loop.  The working G
+// rows take two more VGPRs: seven waves per SIMD, without spills.
+extern "C" __global__ void __launch_bounds__(256, 7) k_sweep_packed_all(SweepArgs A) {
+    sweep_body<true, true>(A);
+}
 
 // Histogram of the Hamming weight |x| over the 2^k combinations
 // x = sum_j i_j L_j of the rows of L (decompose's Z_L, probability.c:

@@ -463,3 +463,28 @@ def test_exact_t24_workload_subsample(E):
     # the total is real: conjugate pairs and real diagonal terms
     assert res[0].im == 0.0 and res[0].re >= 0.0
     assert abs(res[0].kahan_re - res[0].re) <= 1e-5 * max(1.0, res[0].re)
+
+
+@pytest.mark.parametrize("chunk", [512, 1000])
+def test_exact_item_sizes_give_identical_terms(E, chunk):
+    # items of 512 j (exact mode's default for large launches) and of an
+    # unaligned 1000 j split the same t=24, k=11 workload differently; every
+    # pair term and the exact total are the same as with 256
+    from circuitsimulator_amd.engine import Instance, Projector
+    from circuitsimulator_amd.workloads import commuting_paulis, random_L
+    rng = np.random.default_rng(1002)
+    t, r, k = 24, 12, 11
+    ph, xs, zs = commuting_paulis(rng, t, r)
+    L = random_L(rng, t, k)
+    inst = Instance()
+    inst.P = Projector.make(ph, xs, zs, t)
+    inst.exact = 0
+    inst.k = k
+    for i, v in enumerate(L):
+        inst.L[i] = v
+    chi = 1 << k
+    nl = chi * (chi + 1) // 2
+    base, terms256, _ = E.batch_exact([inst], chunk=256, want_terms=True, nterms=nl)
+    res, terms, _ = E.batch_exact([inst], chunk=chunk, want_terms=True, nterms=nl)
+    assert np.array_equal(terms, terms256)
+    assert (res[0].re, res[0].im, res[0].ips) == (base[0].re, base[0].im, base[0].ips)

@@ -14,8 +14,10 @@ This file restates that path in Python register for register: phiL_batch
 tables of x_j and the term-independent part of the W word m * 2^29 |
 16 np + 1, fixation with one edge ballot, restrictions as constrained steps
 with the second column updated in scalar registers, the loop, W parked and
-unpacked later), and the fallback of records with
-more non-pivot columns (restrictions in C++, then the same loop).  Every term
+unpacked later), its exact-mode form (every non-pivot column outside
+supp(x_j) a restriction, columns from working G rows), and the fallback of
+records with more non-pivot columns (restrictions in C++, then the same
+loop).  Every term
 is checked against the oracle's innerProduct (libcirc/stabilizer/
 stabilizer.c:589-667), and the step kinds are counted so that every branch
 is known to be exercised.
@@ -285,6 +287,56 @@ def batch_ip(C, xj, count):
     return W if s56 == END else 0
 
 
+def batch_ip_gen(C, r, xj, count):
+    """One term of phiL_batch<true> (the exact-mode sweep), register for
+    register: fixation as in batch_ip, then every non-pivot column outside
+    supp(x_j) in ascending order as a restriction whose column comes from the
+    working G rows (reset from the record's at the first one) and whose h bit
+    from the working h; an elimination of i over S also takes G_b ^= G_i on
+    the rows of S (row i itself becomes 0; it is retired) and h ^= G_i when
+    c = 1.  Returns the parked W word."""
+    xj, T = table_entry(C, xj)
+    J0 = C["J0"]
+    B = C["actH"] & ~xj
+    act = C["actR"] & xj
+    J, E = [0] * 64, 0
+    for x in range(64):
+        jb = J0[x] & B
+        E |= (popc(jb & ((1 << x) - 1)) & 1) << x
+        J[x] = (J0[x] + ((popc(jb) & 1) << 63)) & U64
+    W = (T + (popc((E ^ C["D2"]) & B) << 31)) & U32
+    pend = C["nonact"] & ~xj
+    if pend:
+        G, h = r.G[:], r.h
+        while pend:
+            q = low(pend)
+            pend &= pend - 1
+            count["rg"] += 1
+            col = sum(1 << x for x in range(64) if (G[x] >> q) & 1)
+            S = col & act
+            c = (popc(col & B) ^ (h >> q)) & 1
+            if not S:
+                if c:
+                    count["rg_vanish"] += 1
+                    return 0
+                count["rg_empty"] += 1
+                continue
+            i = low(S)
+            act &= ~(1 << i)
+            Gi = G[i]
+            for x in bits(S):
+                G[x] ^= Gi
+            if c:
+                count["rg_c"] += 1
+                h ^= Gi
+            if (J[i] >> 62) & 1:
+                count["rg_jii"] += 1
+            W = restrict_step(J, S, c, W)
+            count["rg_elim"] += 1
+    s56, W = elim_loop(J, act, W, count)
+    return W if s56 == END else 0
+
+
 def fallback_ip(r, xj, n, count):
     """ip_phiL<true> with as_expsum_packed: the path of records with three or
     more non-pivot columns (and the layout the exact-mode terms use)."""
@@ -360,7 +412,7 @@ def run_model(t, r_gens, k, seed, nth, nj, only_fallback=False):
         if batch and only_fallback:
             continue
         count["batch" if batch else "fallback"] += 1
-        C = batch_consts(rec, t) if batch else None
+        C = batch_consts(rec, t)
         for j in range(nj):
             xj = xj_of(j, L, k)
             e, p, m = orc.inner(th, phis[j])
@@ -370,7 +422,39 @@ def run_model(t, r_gens, k, seed, nth, nj, only_fallback=False):
             if batch:
                 got = decode(unpack(batch_ip(C, xj, count)))
                 assert got == want, ("batch", t, st - 1, j, got, want)
+            got = decode(unpack(batch_ip_gen(C, rec, xj, count)))
+            assert got == want, ("general", t, st - 1, j, got, want)
             checked += 1
+    return count
+
+
+def run_exact_model(t, r_gens, k, seed, ni, nj):
+    """Exact-mode records: theta'_i = Pi prepL(i) against prepL(j), the
+    records with about t/2 non-pivot columns that the exact-mode sweep runs
+    through the general restriction loop."""
+    from circuitsimulator_amd.workloads import commuting_paulis, random_L
+    rng = np.random.default_rng(seed)
+    ph, xs, zs = commuting_paulis(rng, t, r_gens)
+    L = random_L(rng, t, k)
+    Po = orc.make_projector(ph, xs, zs, t)
+    count = collections.Counter()
+    chi = 1 << k
+    js = sorted(set(int(v) for v in rng.integers(0, chi, size=nj)))
+    phis = {j: orc.prepL(j, t, L) for j in js}
+    for i in sorted(set(int(v) for v in rng.integers(0, chi, size=ni))):
+        th = orc.prepL(i, t, L)
+        if orc.project(th, Po) < 0:
+            continue
+        rec = to_record(th)
+        C = batch_consts(rec, t)
+        count["records"] += 1
+        for j in js:
+            e, p, m = orc.inner(th, phis[j])
+            want = (e, p, m % 8) if e else (0, 0, 0)
+            xj = xj_of(j, L, k)
+            got = decode(unpack(batch_ip_gen(C, rec, xj, count)))
+            assert got == want, ("general", t, i, j, got, want)
+            assert decode(fallback_ip(rec, xj, t, count)) == want, ("fallback", t, i, j)
     return count
 
 
@@ -388,6 +472,25 @@ def test_every_branch_is_exercised():
     # records with three or more non-pivot columns take the fallback path
     count += run_model(12, 6, 6, 14, nth=2, nj=24, only_fallback=True)
     for kind in ("A", "B", "B0", "B1", "C", "vanish", "restrict", "restrict2", "batch", "fallback"):
+        assert count[kind] > 0, (kind, dict(count))
+
+
+@pytest.mark.parametrize("t,r,k,seed", [(24, 12, 6, 21), (40, 10, 5, 22), (60, 20, 4, 23), (9, 2, 4, 24),
+                                        (48, 1, 6, 25)])
+def test_exact_mode_records_general_restrictions(t, r, k, seed):
+    count = run_exact_model(t, r, k, seed, ni=10, nj=12)
+    assert count["records"] > 0 and count["rg"] > 0
+
+
+def test_general_restriction_branches():
+    count = collections.Counter()
+    for t, r, k, seed in [(24, 12, 6, 31), (40, 10, 5, 32), (16, 4, 8, 33), (30, 15, 6, 34)]:
+        count += run_exact_model(t, r, k, seed, ni=12, nj=16)
+    # a restriction whose constraint is empty and unsatisfiable needs the
+    # random states' records
+    for t, r, k, seed in [(12, 3, 6, 12), (30, 4, 10, 13), (20, 2, 8, 15), (16, 1, 10, 16)]:
+        count += run_model(t, r, k, seed, nth=3, nj=24)
+    for kind in ("rg", "rg_elim", "rg_c", "rg_jii", "rg_empty", "rg_vanish", "A", "B", "C"):
         assert count[kind] > 0, (kind, dict(count))
 
 
@@ -414,6 +517,7 @@ def test_exponent_above_64_on_diagonal_terms():
         assert decode(fallback_ip(rec, xj, t, count)) == want, i
         if popc(((1 << t) - 1) & ~rec.act) <= 2:
             assert decode(unpack(batch_ip(batch_consts(rec, t), xj, count))) == want, i
+        assert decode(unpack(batch_ip_gen(batch_consts(rec, t), rec, xj, count))) == want, i
         if e:
             pmax = max(pmax, rec.k + popc(xj) + p)  # n0 + nC of the sweep
     # the case that needs the exponent field's base above 64
