@@ -606,11 +606,11 @@ DEV int phiL_unpack(int w) {
     "v_and_b32 %[vt0], s52, %[glo]\n\t" \
     "v_and_or_b32 %[vt0], s53, %[ghi], %[vt0]\n\t" \
     "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t" \
-    "s_and_b64 s[64:65], s[52:53], s[58:59]\n\t" \
-    "s_and_b64 s[52:53], s[52:53], s[44:45]\n\t" \
-    "s_bcnt1_i32_b64 s51, s[52:53]\n\t" \
+    /* bit 0 of s51: c = h_q + |col & B|; then S = col & act (SCC) */ \
+    "s_and_b64 s[46:47], s[52:53], s[44:45]\n\t" \
+    "s_bcnt1_i32_b64 s51, s[46:47]\n\t" \
     "s_xor_b32 s51, s51, s66\n\t" \
-    "s_cmp_lg_u64 s[64:65], 0\n\t" \
+    "s_and_b64 s[64:65], s[52:53], s[58:59]\n\t" \
     "s_cbranch_scc0 L_rempty%=\n\t" \
     "s_ff1_i32_b64 s56, s[64:65]\n\t" \
     "s_bitset0_b64 s[58:59], s56\n\t" \
