@@ -14,12 +14,13 @@ import numpy as np
 
 from .compilecirc import standardGate
 
-# conjugation tables (x, z) -> (phase increment, x', z') (gadgetize.py:119-123)
-_TABLE = {
-    "H": {(0, 0): (0, 0, 0), (1, 0): (0, 0, 1), (0, 1): (0, 1, 0), (1, 1): (2, 1, 1)},
-    "X": {(0, 0): (0, 0, 0), (1, 0): (0, 1, 0), (0, 1): (2, 0, 1), (1, 1): (2, 1, 1)},
-    "S": {(0, 0): (0, 0, 0), (1, 0): (1, 1, 1), (0, 1): (0, 0, 1), (1, 1): (1, 1, 0)},
-}
+import functools
+
+# The conjugation tables of gadgetize.py:119-123, (x, z) -> (phase increment,
+# x', z'), as updates of bitplanes over all generators at once: H swaps x and
+# z and adds 2 where both are set; X adds 2 where z is set; S adds x to the
+# phase and z ^= x; CX (gadgetize.py:146-156) takes x_t ^= x_c, z_c ^= z_t.
+_SINGLE = ("H", "X", "S")
 
 
 def countY(circuit):
@@ -56,63 +57,97 @@ def countY(circuit):
     return ts, measured, conditional
 
 
-def _conjugate(gate, qubits, phase, x, z):
-    """Conjugate one generator (phase, x, z) by a gate (gadgetize.py:125-156).
-    x and z are lists of 0/1, changed in place."""
-    if len(gate) == 1:
-        q = qubits[0]
-        dph, x[q], z[q] = _TABLE[gate][(int(x[q]), int(z[q]))]
-        return (phase + dph) % 4
-    c, tq = qubits
-    # CX: X_c -> X_c X_t, Z_t -> Z_c Z_t, no phase in the ZX order
-    xc, zc, xt, zt = int(x[c]), int(z[c]), int(x[tq]), int(z[tq])
-    x[c], z[c] = xc, zc ^ zt
-    x[tq], z[tq] = xc ^ xt, zt
-    return phase
+@functools.lru_cache(maxsize=16)
+def _steps(circuit):
+    """The non-empty lines of circuit from last to first, each as (the index
+    of its M or -1, gate, qubits)."""
+    out = []
+    for line in reversed(circuit.splitlines()):
+        if line == "":
+            continue
+        groups = standardGate(line, lineMode=True)
+        gate = standardGate(line)
+        out.append((groups.index("M") if "M" in groups else -1, gate,
+                    tuple(groups.index(letter) for letter in gate)))
+    return tuple(out)
+
+
+class _Planes:
+    """Generators i^phase Z(z) X(x) as bitplanes: bit g of X[q] (Z[q]) is
+    generator g's x (z) on qubit q, bit g of P0 + 2 P1 its phase."""
+
+    def __init__(self, size):
+        self.X = [0] * size
+        self.Z = [0] * size
+        self.P0 = self.P1 = 0
+
+    def add_phase(self, m):
+        self.P1 ^= self.P0 & m
+        self.P0 ^= m
+
+    def single(self, gate, q):
+        X, Z = self.X, self.Z
+        if gate == "H":
+            X[q], Z[q] = Z[q], X[q]
+            self.P1 ^= X[q] & Z[q]
+        elif gate == "X":
+            self.P1 ^= Z[q]
+        elif gate == "S":
+            self.add_phase(X[q])
+            Z[q] ^= X[q]
+        else:
+            raise KeyError(gate)
+
+    def cx(self, c, t):
+        self.X[t] ^= self.X[c]
+        self.Z[c] ^= self.Z[t]
+
+
+def _bit_rows(cols, ng):
+    """(ng, len(cols)) 0/1 int64 matrix, column q the bits of cols[q]."""
+    nb = max(1, (ng + 7) // 8)
+    raw = b"".join(c.to_bytes(nb, "little") for c in cols)
+    bits = np.unpackbits(np.frombuffer(raw, dtype=np.uint8).reshape(len(cols), nb), axis=1,
+                         bitorder="little")[:, :ng]
+    return bits.T.astype(np.int64)
 
 
 def gadgetize(circuit, Mdict, y):
     """Projector (phases, xs, zs) of the outcomes Mdict on the circuit's
     qubits and y on the T ancillas, in the qubit order of the circuit
-    followed by the ancillas (gadgetize.py:66-181)."""
+    followed by the ancillas (gadgetize.py:66-181).  Every generator is
+    conjugated backwards through the circuit, all generators at once."""
     n = len(standardGate(circuit.split("\n", 1)[0], lineMode=True))
     size = n + len(y)
-    phases, xs, zs = [], [], []
-
-    def z_generator(q, outcome):
-        xs.append([0] * size)
-        z = [0] * size
-        z[q] = 1
-        zs.append(z)
-        phases.append(2 if outcome == 1 else 0)
-
-    for q in Mdict.keys():
-        z_generator(q, Mdict[q])
-    for q in range(n, size):
-        z_generator(q, y[q - n])
+    gens = [(q, Mdict[q]) for q in Mdict.keys()] + [(q, y[q - n]) for q in range(n, size)]
+    ng = len(gens)
+    pl = _Planes(size)
+    for g, (q, outcome) in enumerate(gens):
+        pl.Z[q] |= 1 << g
+        if outcome == 1:
+            pl.P1 |= 1 << g
 
     tpos = n
-    for line in reversed(circuit.splitlines()):
-        if line == "":
-            continue
-        groups = standardGate(line, lineMode=True)
-        if "M" in groups and Mdict[groups.index("M")] == 0:
+    for m, gate, qubits in _steps(circuit):
+        if m >= 0 and Mdict[m] == 0:
             continue  # the gate runs only on outcome 1
-        gate = standardGate(line)
-        qubits = [groups.index(letter) for letter in gate]
-        for g in range(len(phases)):
+        if ng:
             if gate == "T":
-                ph = _conjugate("CX", [qubits[0], tpos], phases[g], xs[g], zs[g])
+                pl.cx(qubits[0], tpos)
                 if y[tpos - n] == 1:
-                    ph = _conjugate("S", qubits, ph, xs[g], zs[g])
+                    pl.single("S", qubits[0])
                 for h in ("H", "S", "S", "S"):
-                    ph = _conjugate(h, [tpos], ph, xs[g], zs[g])
-                phases[g] = ph
+                    pl.single(h, tpos)
+            elif len(gate) == 1:
+                pl.single(gate, qubits[0])
             else:
-                phases[g] = _conjugate(gate, qubits, phases[g], xs[g], zs[g])
+                c, tq = qubits
+                pl.cx(c, tq)
         if gate == "T":
             tpos += 1
-    return phases, [np.array(v) for v in xs], [np.array(v) for v in zs]
+    phases = [((pl.P0 >> g) & 1) + 2 * ((pl.P1 >> g) & 1) for g in range(ng)]
+    xr, zr = _bit_rows(pl.X, ng), _bit_rows(pl.Z, ng)
+    return phases, [xr[g] for g in range(ng)], [zr[g] for g in range(ng)]
 
 
 def _product_phase(x1, z1, x2, z2):

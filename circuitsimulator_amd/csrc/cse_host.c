@@ -355,3 +355,22 @@ int cse_decompose(int t, uint64_t seed, uint64_t *L, double *norm, int *exact, i
     if (fidelity) *norm = sqrt(pow(2, *k) * Z_L);
     return CSE_OK;
 }
+
+int cse_decompose_many(int t, const uint64_t *seeds, int n, uint64_t *L, int ldl, double *norm, int *exact, int *k,
+                       double fidbound, int rank, int fidelity, int forceL, int verbose, int quiet)
+{
+    if (!seeds || !L || !norm || !exact || !k || n < 0 || ldl < 1) return CSE_ERR_ARG;
+    const int exact0 = *exact, k0 = *k;
+    for (int i = 0; i < n; i++) {
+        uint64_t rows[CSE_NMAX];
+        *exact = exact0;
+        *k = k0;
+        const int rc = cse_decompose(t, seeds[i], rows, &norm[i], exact, k, fidbound, rank, fidelity, forceL,
+                                     verbose, quiet);
+        if (rc != CSE_OK) return rc;
+        const int nr = *exact ? 0 : *k;
+        if (nr > ldl) return CSE_ERR_ARG;
+        for (int r = 0; r < nr; r++) L[(size_t)i * (size_t)ldl + (size_t)r] = rows[r];
+    }
+    return CSE_OK;
+}

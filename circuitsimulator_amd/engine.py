@@ -24,7 +24,7 @@ ABI_SYMBOLS = [
     "cse_create", "cse_destroy", "cse_error", "cse_device_info", "cse_inner_products",
     "cse_prep_states", "cse_random_states", "cse_exact_projector", "cse_sampled_projector",
     "cse_batch_exact", "cse_decompose", "cse_hist_value", "cse_sampling_kept", "cse_projector_norm",
-    "cse_lspace_weights", "cse_batch_sampled",
+    "cse_lspace_weights", "cse_batch_sampled", "cse_decompose_many",
 ]
 ERR_NODEV = -5
 
@@ -120,6 +120,9 @@ def lib():
         L.cse_decompose.argtypes = [ctypes.c_int, U64, P(U64), P(ctypes.c_double), P(ctypes.c_int),
                                     P(ctypes.c_int), ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int]
+        L.cse_decompose_many.argtypes = [ctypes.c_int, P(U64), ctypes.c_int, P(U64), ctypes.c_int,
+                                         P(ctypes.c_double), P(ctypes.c_int), P(ctypes.c_int), ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.cse_hist_value.argtypes = [P(ctypes.c_int64), ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]
         L.cse_sampling_kept.argtypes = [ctypes.c_int] * 7
         L.cse_lspace_weights.argtypes = [ctypes.c_void_p, P(U64), ctypes.c_int, P(ctypes.c_int64)]

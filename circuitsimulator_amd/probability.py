@@ -85,30 +85,31 @@ def decompose(t, config, seed=0):
 
 
 def decompose_many(t, config, seeds):
-    """decompose once per seed with the same config: (L rows as a uint64
-    array of shape (len(seeds), k), or None for the exact decomposition,
-    norm, exact, k).  exact, k and norm depend on t and config only."""
+    """decompose once per seed with the same config, in one call
+    (cse_decompose_many): (L rows as a uint64 array of shape (len(seeds), k),
+    or None for the exact decomposition, norm, exact, k).  exact and k depend
+    on t and config only; norm is the last seed's (it differs between seeds
+    only with the fidelity option)."""
     import ctypes
     from .engine import NMAX, U64
-    L = (U64 * NMAX)()
-    norm = ctypes.c_double()
-    args = (float(config.get("fidbound") or 1e-5), 1 if config.get("rank") else 0,
-            1 if config.get("fidelity") else 0, 1 if config.get("forceL") else 0,
-            1 if config.get("verbose") else 0, 1 if config.get("quiet") else 0)
-    f = lib().cse_decompose
-    out, ex0, k0 = None, None, None
-    for n, seed in enumerate(seeds):
-        exact = ctypes.c_int(1 if config.get("exact") else 0)
-        k = ctypes.c_int(int(config.get("k") or 0))
-        if f(int(t), int(seed) & _SEED_MASK, L, ctypes.byref(norm), ctypes.byref(exact), ctypes.byref(k), *args):
-            raise ValueError("decomposition of t=%d is outside the engine's range" % t)
-        if out is None:
-            ex0, k0 = bool(exact.value), k.value
-            if ex0:
-                return None, norm.value, True, k0
-            out = np.zeros((len(seeds), k0), dtype=np.uint64)
-        out[n] = np.ctypeslib.as_array(L)[:k0]
-    return out, norm.value, ex0, k0
+    n = len(seeds)
+    sd = np.array([int(s) & _SEED_MASK for s in seeds], dtype=np.uint64)
+    L = np.zeros((max(n, 1), NMAX), dtype=np.uint64)
+    norms = np.zeros(max(n, 1))
+    exact = ctypes.c_int(1 if config.get("exact") else 0)
+    k = ctypes.c_int(int(config.get("k") or 0))
+    rc = lib().cse_decompose_many(int(t), sd.ctypes.data_as(ctypes.POINTER(U64)), n,
+                                  L.ctypes.data_as(ctypes.POINTER(U64)), NMAX,
+                                  norms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(exact),
+                                  ctypes.byref(k), float(config.get("fidbound") or 1e-5),
+                                  1 if config.get("rank") else 0, 1 if config.get("fidelity") else 0,
+                                  1 if config.get("forceL") else 0, 1 if config.get("verbose") else 0,
+                                  1 if config.get("quiet") else 0)
+    if rc != 0:
+        raise ValueError("decomposition of t=%d is outside the engine's range" % t)
+    if exact.value:
+        return None, float(norms[n - 1]), True, k.value
+    return np.ascontiguousarray(L[:n, :k.value]), float(norms[n - 1]), False, k.value
 
 
 def _same(Gp, Hp):

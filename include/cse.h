@@ -196,6 +196,16 @@ int cse_decompose(int t, uint64_t seed, uint64_t *L, double *norm, int *exact, i
                   double fidbound, int rank, int fidelity, int forceL, int verbose, int quiet);
 
 /*
+ * cse_decompose once per seed, with the same t and options: the per-call
+ * decompositions of a batch of probability evaluations (probability.c:
+ * 307-415, one seed per call).  Seed i's rows go to L + i * ldl, its norm to
+ * norm[i]; *exact and *k are in/out as in cse_decompose (they depend on t and
+ * the options only).  CSE_ERR_ARG if k exceeds ldl.
+ */
+int cse_decompose_many(int t, const uint64_t *seeds, int n, uint64_t *L, int ldl, double *norm, int *exact, int *k,
+                       double fidbound, int rank, int fidelity, int forceL, int verbose, int quiet);
+
+/*
  * Weights of the row space of L for decompose's fidelity Z_L
  * (probability.c:371-410): hist[w] = #{i in [0, 2^k) : |sum_j i_j L_j| = w},
  * w = 0..64 (65 entries), on the device.  k <= 40.

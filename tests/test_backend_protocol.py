@@ -185,6 +185,25 @@ def test_backend_sampled_mode_matches_engine():
     assert num == pytest.approx(res.mean, rel=1e-15)
 
 
+def test_decompose_many_equals_per_seed_calls():
+    # one cse_decompose_many call per batch of seeds gives each seed's
+    # decomposition exactly as its own cse_decompose call
+    from circuitsimulator_amd.probability import decompose, decompose_many
+    seeds = [3, 2 ** 61 + 5, 977, 12345678901]
+    for t, cfg in [(40, {"k": 8}), (10, {}), (30, {"k": 6, "rank": True}), (24, {"k": 5, "fidelity": True}),
+                   (16, {"exact": True}), (5, {"k": 9})]:
+        cfg = dict(cfg, quiet=True, fidbound=1e-5)
+        Ls, norm, ex, k = decompose_many(t, cfg, seeds)
+        for i, sd in enumerate(seeds):
+            L1, n1, e1, k1 = decompose(t, cfg, sd)
+            assert (e1, k1) == (ex, k), (t, cfg)
+            if ex:
+                assert Ls is None
+            else:
+                assert [int(v) for v in Ls[i]] == L1, (t, cfg, sd)
+        assert norm == decompose(t, cfg, seeds[-1])[1]
+
+
 def _row_space_weights(L, k):
     xs = np.zeros(1, dtype=np.uint64)
     for row in L[:k]:

@@ -185,6 +185,94 @@ def random_clifford_t(rng, n, ngates, nt):
     return "\n".join(gates) + "\n"
 
 
+# gadgetize.py:119-123: (x, z) -> (phase increment, x', z') per qubit
+_REF_TABLES = {
+    "H": {(0, 0): (0, 0, 0), (1, 0): (0, 0, 1), (0, 1): (0, 1, 0), (1, 1): (2, 1, 1)},
+    "X": {(0, 0): (0, 0, 0), (1, 0): (0, 1, 0), (0, 1): (2, 0, 1), (1, 1): (2, 1, 1)},
+    "S": {(0, 0): (0, 0, 0), (1, 0): (1, 1, 1), (0, 1): (0, 0, 1), (1, 1): (1, 1, 0)},
+}
+
+
+def _gadgetize_per_generator(circ, Mdict, y):
+    """gadgetize.py:66-181 one generator at a time through the tables: the
+    independent check of the bitplane conjugation."""
+    from circuitsimulator_amd.compile.compilecirc import standardGate
+    n = len(standardGate(circ.split("\n", 1)[0], lineMode=True))
+    size = n + len(y)
+    gens = [(q, Mdict[q]) for q in Mdict] + [(q, y[q - n]) for q in range(n, size)]
+    out = []
+    for q0, outcome in gens:
+        ph, x, z = 2 * outcome, [0] * size, [0] * size
+        z[q0] = 1
+
+        def one(gate, q):
+            nonlocal ph
+            d, x[q], z[q] = _REF_TABLES[gate][(x[q], z[q])]
+            ph = (ph + d) % 4
+
+        def cx(c, t):
+            z[c] ^= z[t]
+            x[t] ^= x[c]
+
+        tpos = n
+        for line in reversed(circ.splitlines()):
+            if line == "":
+                continue
+            groups = standardGate(line, lineMode=True)
+            if "M" in groups and Mdict[groups.index("M")] == 0:
+                continue
+            gate = standardGate(line)
+            qs = [groups.index(c) for c in gate]
+            if gate == "T":
+                cx(qs[0], tpos)
+                if y[tpos - n]:
+                    one("S", qs[0])
+                for h in "HSSS":
+                    one(h, tpos)
+                tpos += 1
+            elif len(gate) == 1:
+                one(gate, qs[0])
+            else:
+                cx(*qs)
+        out.append((ph, x, z))
+    return out
+
+
+def test_gadgetize_bitplanes_match_per_generator_tables():
+    rng = np.random.default_rng(77)
+    circs = ["H__\nT__\nCX_\n_CX\n__T\n_S_\n", "X_\nT_\nH_\n", "_H\nCX\nHM\nH_\nTM\nX_\n"]
+    for _ in range(12):
+        n = int(rng.integers(2, 7))
+        while True:
+            c = random_clifford_t(rng, n, int(rng.integers(10, 40)), int(rng.integers(0, 6)))
+            lines = c.splitlines()
+            for _ in range(int(rng.integers(0, 4))):
+                ln = ["_"] * n
+                ln[int(rng.integers(0, n))] = "X"
+                lines.insert(int(rng.integers(0, len(lines) + 1)), "".join(ln))
+            c = "\n".join(lines) + "\n"
+            if valid_ts(c):
+                break
+        circs.append(c)
+    for circ in circs:
+        t, Ms, MTs = gadgetize.countY(circ)
+        n = len(circ.splitlines()[0])
+        for trial in range(3):
+            qs = [q for q in range(n) if q not in Ms]
+            meas = {int(q): int(rng.integers(0, 2)) for q in rng.permutation(qs)[: int(rng.integers(0, len(qs) + 1))]}
+            for q in Ms:
+                meas[q] = int(rng.integers(0, 2))
+            # a T conditioned on outcome 1 takes an ancilla too (projectors.py:40-42)
+            nt = t + sum(1 for q in MTs if meas[q] == 1)
+            y = [int(b) for b in rng.integers(0, 2, size=nt)]
+            ph, xs, zs = gadgetize.gadgetize(circ, meas, y)
+            want = _gadgetize_per_generator(circ, meas, y)
+            assert len(ph) == len(want)
+            for g, (wp, wx, wz) in enumerate(want):
+                assert ph[g] == wp and list(xs[g]) == wx and list(zs[g]) == wz, (circ, meas, y, g)
+                assert xs[g].dtype == np.int64 and zs[g].dtype == np.int64
+
+
 def valid_ts(circ):
     try:
         gadgetize.countY(circ)
