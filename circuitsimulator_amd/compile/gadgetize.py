@@ -28,6 +28,12 @@ def countY(circuit):
     order of first measurement), and the measured qubits that condition a T
     gate (gadgetize.py:14-61).  Rejects operations on measured qubits and
     T T on one qubit with nothing in between."""
+    ts, measured, conditional = _count(circuit)
+    return ts, list(measured), list(conditional)
+
+
+@functools.lru_cache(maxsize=16)
+def _count(circuit):
     ts = 0
     pending_t = []  # qubits whose last gate was a T
     measured = []
@@ -54,7 +60,7 @@ def countY(circuit):
                 pending_t.remove(q)
         if "T" in groups and "M" in groups:
             conditional.append(groups.index("M"))
-    return ts, measured, conditional
+    return ts, tuple(measured), tuple(conditional)
 
 
 @functools.lru_cache(maxsize=16)

@@ -455,86 +455,90 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
 // s[58:59] act, s[62:63] J_a, s[64:65] eta, s[66:67] Jc, v40 = -2 << 29.  A
 // VALU instruction reads an SGPR that v_readlane wrote only two or more
 // instructions later.  Every step clears a bit of act below 60.
+// CSE_ELIM_LOOP_BODY(S, CNEXT) is the loop with its labels suffixed by S
+// (two copies can share an asm block) and CNEXT the instruction that
+// follows a step without a constraint (the branch back to the top).
 #define CSE_END_ROW 60
-#define CSE_ELIM_LOOP                                                  \
-    "L_top%=:\n\t"                                                     \
-    "s_ff1_i32_b64 s56, s[58:59]\n\t"                                  \
-    "s_bitset0_b64 s[58:59], s56\n\t"                                  \
-    "v_readlane_b32 s62, %[jlo], s56\n\t"                              \
-    "v_readlane_b32 s63, %[jhi], s56\n\t"                              \
-    "s_and_b64 s[64:65], s[62:63], s[58:59]\n\t"                       \
-    "s_bitcmp1_b32 s63, 30\n\t"                                        \
-    "s_cbranch_scc0 L_nonA%=\n\t"                                      \
-    /* D_a in {2,6}: rows in eta take J ^= eta, D -= D_a; m += 1       \
-       (D_a = 2) or -1 (D_a = 6) */                                    \
-    "v_xor_b32 %[vt0], s64, %[jlo]\n\t"                                \
-    "v_xad_u32 %[vt1], %[jhi], s65, v40\n\t"                           \
-    "v_cndmask_b32_e64 %[jlo], %[jlo], %[vt0], s[64:65]\n\t"           \
-    "v_cndmask_b32_e64 %[jhi], %[jhi], %[vt1], s[64:65]\n\t"           \
-    "s_bitcmp1_b32 s63, 31\n\t"                                        \
-    "s_cbranch_scc1 L_a6%=\n\t"                                        \
-    "s_add_u32 s60, s60, 0x20000000\n\t"                               \
-    "s_branch L_top%=\n"                                               \
-    "L_a6%=:\n\t"                                                      \
-    "s_add_u32 s60, s60, 0xe0000000\n\t"                               \
-    "s_branch L_top%=\n"                                               \
-    "L_nonA%=:\n\t"                                                    \
-    /* eta without bit 63: a constraint? */                            \
-    "s_lshl_b64 s[66:67], s[64:65], 1\n\t"                             \
-    "s_cbranch_scc0 L_C%=\n\t"                                         \
-    /* D_a in {0,4} with a constraint: pivot i = min(eta).  Row i and  \
-       column i are dead from here on, so eta itself serves as Sp;     \
-       its bit 63 is c = D_a / 4, so D += 4 c on the rows of M comes   \
-       with the XOR of Sp.  bitop3 0x78 is S0 ^ (S1 & S2), 0x60 is     \
-       S0 & (S1 ^ S2). */                                              \
-    "s_ff1_i32_b64 s56, s[64:65]\n\t"                                  \
-    "s_bitset0_b64 s[58:59], s56\n\t"                                  \
-    "v_readlane_b32 s66, %[jlo], s56\n\t"                              \
-    "v_readlane_b32 s67, %[jhi], s56\n\t"                              \
-    "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t"                    \
-    "s_and_b32 s56, s67, 0xe0000000\n\t"                               \
-    "s_bitcmp1_b32 s67, 30\n\t"                                        \
-    "s_cbranch_scc1 L_jii%=\n\t"                                       \
-    /* D_i in {0,4}: M = Jc, and D += D_i on Sp is bit 63 of Jc. Rows  \
-       of M take J ^= Sp with bit 63 flipped where x is in Sp too      \
-       (the 4 J_xi of the rows in Sp). */                              \
-    "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"                    \
-    "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"         \
-    "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"         \
-    "v_and_b32 %[vt1], 0x80000000, %[vt0]\n\t"                         \
-    "v_xor_b32 %[vt1], s65, %[vt1]\n\t"                                \
-    "s_bitcmp1_b32 s63, 31\n\t"                                        \
-    "s_cbranch_scc0 L_noc0%=\n\t"                                      \
-    "s_add_u32 s60, s60, s56\n"                                        \
-    "L_noc0%=:\n\t"                                                    \
-    "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"         \
-    "v_bitop3_b32 %[jhi], %[jhi], %[vt1], %[vt3] bitop3:0x78\n\t"      \
-    "s_branch L_top%=\n"                                               \
-    /* D_i in {2,6}: M = Jc ^ Sp; rows in Sp take J ^= Jc and          \
-       D += D_i + 4 J_xi */                                            \
-    "L_jii%=:\n\t"                                                     \
-    "s_and_b32 s67, s67, 0x1fffffff\n\t"                               \
-    "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"                    \
-    "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"         \
-    "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"         \
-    "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t"                         \
-    "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t"         \
-    "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t"                             \
-    "v_xor_b32 %[vt3], %[vt3], %[vt0]\n\t"                             \
-    "s_bitcmp1_b32 s63, 31\n\t"                                        \
-    "s_cbranch_scc0 L_noc1%=\n\t"                                      \
-    "s_add_u32 s60, s60, s56\n"                                        \
-    "L_noc1%=:\n\t"                                                    \
-    "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"         \
-    "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t"         \
-    "s_branch L_top%=\n"                                               \
-    /* D_a in {0,4} without a constraint: the sum vanishes (D_a = 4,   \
-       or the end row) or nC += 1 */                                   \
-    "L_C%=:\n\t"                                                       \
-    "s_bitcmp1_b32 s63, 31\n\t"                                        \
-    "s_cbranch_scc1 L_end%=\n\t"                                       \
-    "s_add_u32 s60, s60, 0xfffffff0\n\t"                               \
-    "s_branch L_top%=\n"
+#define CSE_ELIM_LOOP_BODY(S, CNEXT)                                        \
+    "L_top" S "%=:\n\t"                                                     \
+    "s_ff1_i32_b64 s56, s[58:59]\n\t"                                       \
+    "s_bitset0_b64 s[58:59], s56\n\t"                                       \
+    "v_readlane_b32 s62, %[jlo], s56\n\t"                                   \
+    "v_readlane_b32 s63, %[jhi], s56\n\t"                                   \
+    "s_and_b64 s[64:65], s[62:63], s[58:59]\n\t"                            \
+    "s_bitcmp1_b32 s63, 30\n\t"                                             \
+    "s_cbranch_scc0 L_nonA" S "%=\n\t"                                      \
+    /* D_a in {2,6}: rows in eta take J ^= eta, D -= D_a; m += 1            \
+       (D_a = 2) or -1 (D_a = 6) */                                         \
+    "v_xor_b32 %[vt0], s64, %[jlo]\n\t"                                     \
+    "v_xad_u32 %[vt1], %[jhi], s65, v40\n\t"                                \
+    "v_cndmask_b32_e64 %[jlo], %[jlo], %[vt0], s[64:65]\n\t"                \
+    "v_cndmask_b32_e64 %[jhi], %[jhi], %[vt1], s[64:65]\n\t"                \
+    "s_bitcmp1_b32 s63, 31\n\t"                                             \
+    "s_cbranch_scc1 L_a6" S "%=\n\t"                                        \
+    "s_add_u32 s60, s60, 0x20000000\n\t"                                    \
+    "s_branch L_top" S "%=\n"                                               \
+    "L_a6" S "%=:\n\t"                                                      \
+    "s_add_u32 s60, s60, 0xe0000000\n\t"                                    \
+    "s_branch L_top" S "%=\n"                                               \
+    "L_nonA" S "%=:\n\t"                                                    \
+    /* eta without bit 63: a constraint? */                                 \
+    "s_lshl_b64 s[66:67], s[64:65], 1\n\t"                                  \
+    "s_cbranch_scc0 L_C" S "%=\n\t"                                         \
+    /* D_a in {0,4} with a constraint: pivot i = min(eta).  Row i and       \
+       column i are dead from here on, so eta itself serves as Sp;          \
+       its bit 63 is c = D_a / 4, so D += 4 c on the rows of M comes        \
+       with the XOR of Sp.  bitop3 0x78 is S0 ^ (S1 & S2), 0x60 is          \
+       S0 & (S1 ^ S2). */                                                   \
+    "s_ff1_i32_b64 s56, s[64:65]\n\t"                                       \
+    "s_bitset0_b64 s[58:59], s56\n\t"                                       \
+    "v_readlane_b32 s66, %[jlo], s56\n\t"                                   \
+    "v_readlane_b32 s67, %[jhi], s56\n\t"                                   \
+    "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t"                         \
+    "s_and_b32 s56, s67, 0xe0000000\n\t"                                    \
+    "s_bitcmp1_b32 s67, 30\n\t"                                             \
+    "s_cbranch_scc1 L_jii" S "%=\n\t"                                       \
+    /* D_i in {0,4}: M = Jc, and D += D_i on Sp is bit 63 of Jc. Rows       \
+       of M take J ^= Sp with bit 63 flipped where x is in Sp too           \
+       (the 4 J_xi of the rows in Sp). */                                   \
+    "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"                         \
+    "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"              \
+    "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"              \
+    "v_and_b32 %[vt1], 0x80000000, %[vt0]\n\t"                              \
+    "v_xor_b32 %[vt1], s65, %[vt1]\n\t"                                     \
+    "s_bitcmp1_b32 s63, 31\n\t"                                             \
+    "s_cbranch_scc0 L_noc0" S "%=\n\t"                                      \
+    "s_add_u32 s60, s60, s56\n"                                             \
+    "L_noc0" S "%=:\n\t"                                                    \
+    "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"              \
+    "v_bitop3_b32 %[jhi], %[jhi], %[vt1], %[vt3] bitop3:0x78\n\t"           \
+    "s_branch L_top" S "%=\n"                                               \
+    /* D_i in {2,6}: M = Jc ^ Sp; rows in Sp take J ^= Jc and               \
+       D += D_i + 4 J_xi */                                                 \
+    "L_jii" S "%=:\n\t"                                                     \
+    "s_and_b32 s67, s67, 0x1fffffff\n\t"                                    \
+    "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"                         \
+    "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"              \
+    "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"              \
+    "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t"                              \
+    "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t"              \
+    "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t"                                  \
+    "v_xor_b32 %[vt3], %[vt3], %[vt0]\n\t"                                  \
+    "s_bitcmp1_b32 s63, 31\n\t"                                             \
+    "s_cbranch_scc0 L_noc1" S "%=\n\t"                                      \
+    "s_add_u32 s60, s60, s56\n"                                             \
+    "L_noc1" S "%=:\n\t"                                                    \
+    "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"              \
+    "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t"              \
+    "s_branch L_top" S "%=\n"                                               \
+    /* D_a in {0,4} without a constraint: the sum vanishes (D_a = 4,        \
+       or the end row) or nC += 1 */                                        \
+    "L_C" S "%=:\n\t"                                                       \
+    "s_bitcmp1_b32 s63, 31\n\t"                                             \
+    "s_cbranch_scc1 L_end%=\n\t"                                            \
+    "s_add_u32 s60, s60, 0xfffffff0\n\t"                                    \
+    CNEXT
+#define CSE_ELIM_LOOP CSE_ELIM_LOOP_BODY("", "s_branch L_top%=\n")
 
 // The loop on an AState (records of n <= 60, row 60 the end row); p and m
 // as as_expsum.  W's low field counts down from the base 128 (p = n0 + nC

@@ -478,7 +478,40 @@ DEV int phiL_unpack(int w) {
 // constraint's right side / a temporary, s[52:53] the edge ballot / the
 // column, s54 the table's W, s[62:63] the pending non-pivot columns before
 // the loop.
-#define PHIL_TERM \
+// The elimination loop of exact-mode terms, which have many variables that
+// nothing couples to.  At a term's first step without a constraint (a C
+// step), every active variable whose row meets no active variable (bits
+// below 61 of J_x & act, its own bit included) and whose D is in {0,4}
+// (bit 62 clear; the stored diagonal bit says the same, but the batch does
+// not rely on it) is uncoupled for the rest of the loop: each one is a C
+// step (a factor 2, W -= 16), or with D = 4 the sum vanishes.  They leave
+// act together and a second copy of the loop goes on without the check.
+// The end row and bit 63 are kept out of the batch.
+#define CSE_ELIM_LOOP_ISO \
+    CSE_ELIM_LOOP_BODY("", "s_branch L_iso%=\n") \
+    "L_iso%=:\n\t" \
+    "s_and_b32 s67, s59, 0x1fffffff\n\t" \
+    "s_bitset1_b32 s67, 30\n\t" \
+    "v_and_b32 %[vt0], s58, %[jlo]\n\t" \
+    "v_and_b32 %[vt1], s67, %[jhi]\n\t" \
+    "v_or_b32 %[vt0], %[vt0], %[vt1]\n\t" \
+    "v_cmp_eq_u32_e64 s[64:65], 0, %[vt0]\n\t" \
+    "v_cmp_gt_i32_e64 s[66:67], 0, %[jhi]\n\t" \
+    "s_and_b64 s[64:65], s[64:65], s[58:59]\n\t" \
+    "s_and_b32 s65, s65, 0x6fffffff\n\t" \
+    "s_and_b64 s[66:67], s[66:67], s[64:65]\n\t" \
+    "s_cbranch_scc1 L_isov%=\n\t" \
+    "s_andn2_b64 s[58:59], s[58:59], s[64:65]\n\t" \
+    "s_bcnt1_i32_b64 s66, s[64:65]\n\t" \
+    "s_lshl_b32 s66, s66, 4\n\t" \
+    "s_sub_u32 s60, s60, s66\n\t" \
+    "s_branch L_top2%=\n" \
+    "L_isov%=:\n\t" \
+    "s_mov_b32 s56, 0\n\t" \
+    "s_branch L_end%=\n" \
+    CSE_ELIM_LOOP_BODY("2", "s_branch L_top2%=\n")
+
+#define PHIL_TERM(LOOP) \
     "v_mov_b32 %[parked], 0\n\t" \
     "v_mov_b32 v40, 0xc0000000\n\t" \
     "s_mov_b32 s61, m0\n\t" \
@@ -511,7 +544,7 @@ DEV int phiL_unpack(int w) {
     "s_andn2_b64 s[62:63], %[nonact], s[46:47]\n\t" \
     "s_cbranch_scc1 L_rs%=\n" \
     "L_elim%=:\n" \
-    CSE_ELIM_LOOP \
+    LOOP \
     "L_end%=:\n\t" \
     "s_cmp_eq_u32 s56, 60\n\t" \
     "s_cselect_b32 s60, s60, 0\n" \
@@ -661,7 +694,7 @@ DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, uns
     unsigned jlo, jhi, vt0, vt1, vt2, vt3;
     if constexpr (!ALL) {
         asm volatile(
-            PHIL_TERM
+            PHIL_TERM(CSE_ELIM_LOOP)
             PHIL_RESTRICT2
             "L_done%=:\n\t"
             "s_mov_b32 m0, s61\n\t"
@@ -676,7 +709,7 @@ DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, uns
     } else {
         unsigned glo, ghi;
         asm volatile(
-            PHIL_TERM
+            PHIL_TERM(CSE_ELIM_LOOP_ISO)
             PHIL_RESTRICT_ALL
             "L_done%=:\n\t"
             "s_mov_b32 m0, s61\n\t"

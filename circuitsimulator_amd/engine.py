@@ -7,6 +7,7 @@ present, the calls raise.
 import atexit
 import ctypes
 import os
+import threading
 
 import numpy as np
 
@@ -278,6 +279,31 @@ class Engine:
         self._check(self._L.cse_batch_sampled(self.h, instances.ctypes.data_as(P(SampledInstance)), n, chunk,
                                               res, ctypes.byref(ms)))
         return np.array([r.mean for r in res]), list(res), ms.value, samples
+
+
+# one engine context per (device, host thread) for callers that evaluate many
+# batches (sampleShots): device buffers and pinned staging stay allocated
+# between calls; destroyed at exit
+_shared = {}
+
+
+def shared_engine(device=0):
+    """The process's Engine for this device and host thread, created on first
+    use (a context serves one host thread at a time)."""
+    key = (int(device), threading.get_ident())
+    e = _shared.get(key)
+    if e is None or not e.h:
+        e = _shared[key] = Engine(device)
+    return e
+
+
+def _close_shared():
+    for e in _shared.values():
+        e.close()
+    _shared.clear()
+
+
+atexit.register(_close_shared)
 
 
 def decompose(t, seed, exact=0, k=0, fidbound=1e-5, rank=0, fidelity=0, forceL=0, verbose=0, quiet=1):

@@ -223,7 +223,7 @@ def sampleShots(circ, measure, sample, shots, config={}):
     when the constraints cannot be met).
     """
     from .compile.compilecirc import standardGate
-    from .engine import Engine, instance_array, sampling_kept
+    from .engine import instance_array, sampling_kept, shared_engine
     from .probability import _pick_sampling, decompose_many, projector_norms
 
     t, Ms, _ = _gadgetize.countY(circ)
@@ -304,7 +304,7 @@ def sampleShots(circ, measure, sample, shots, config={}):
                     out[c] = 0.0 if num == 0 else base * num / den
                 return out
             if engine is None:
-                engine = Engine(device)
+                engine = shared_engine(device)
             num = _sampled_medians(engine, PH[todo], xk, zk, ex, k, Ls, seeds, samples, bins, fill, 0)
             if Hm[0]:
                 den = _sampled_medians(engine, np.asarray([Hm[0]]), Hm[1], Hm[2], ex, k, Ls, seeds, samples,
@@ -330,7 +330,7 @@ def sampleShots(circ, measure, sample, shots, config={}):
             if Hm[0]:
                 fill(A, slice(nnum, 2 * nnum), [Hm[0]], Hm[1], Hm[2], False, k, Ls)
         if engine is None:
-            engine = Engine(device)
+            engine = shared_engine(device)
         res, _, _ = engine.batch_exact(A)
         vals = np.array([r.value for r in res])
         num = vals[:nnum][inv]
@@ -353,30 +353,26 @@ def sampleShots(circ, measure, sample, shots, config={}):
         V[:, :m] = bits
         return _round_phases(prods, V), prods[4]
 
-    try:
-        ncon = len(measure)
-        con_bits = np.array([int(measure[q]) for q in order[:ncon]], dtype=np.int64)
-        Psofar = np.ones(shots)
-        if ncon:
-            (PH, xk, zk, imp), uu = round_projectors(ncon, con_bits[None, :])
-            P = evaluate(PH, xk, zk, imp, uu, True)[0]
-            if P == 0:
-                return [""] * shots
-            if P < 1e-5:
-                print("Warning: probability of satisfying qubit constraints very low (%f). Could be impossible." % P)
-            Psofar[:] = P
-        bits = np.zeros((shots, ncon + len(sample)), dtype=np.int64)
-        bits[:, :ncon] = con_bits
-        for r in range(len(sample)):
-            m = ncon + r + 1
-            (PH, xk, zk, imp), uu = round_projectors(m, bits[:, :m])
-            P = evaluate(PH, xk, zk, imp, uu, cfg["exact"])
-            P0 = P / Psofar
-            u01 = rng.random(shots) if rng is not None else np.random.random(shots)
-            draw = np.where(u01 < P0, 0, 1)
-            bits[:, m - 1] = draw
-            Psofar = _next_psofar(cfg.get("chain", "reference"), P, P0, Psofar, draw)
-        return ["".join(str(int(b)) for b in row[ncon:]) for row in bits]
-    finally:
-        if engine is not None:
-            engine.close()
+    ncon = len(measure)
+    con_bits = np.array([int(measure[q]) for q in order[:ncon]], dtype=np.int64)
+    Psofar = np.ones(shots)
+    if ncon:
+        (PH, xk, zk, imp), uu = round_projectors(ncon, con_bits[None, :])
+        P = evaluate(PH, xk, zk, imp, uu, True)[0]
+        if P == 0:
+            return [""] * shots
+        if P < 1e-5:
+            print("Warning: probability of satisfying qubit constraints very low (%f). Could be impossible." % P)
+        Psofar[:] = P
+    bits = np.zeros((shots, ncon + len(sample)), dtype=np.int64)
+    bits[:, :ncon] = con_bits
+    for r in range(len(sample)):
+        m = ncon + r + 1
+        (PH, xk, zk, imp), uu = round_projectors(m, bits[:, :m])
+        P = evaluate(PH, xk, zk, imp, uu, cfg["exact"])
+        P0 = P / Psofar
+        u01 = rng.random(shots) if rng is not None else np.random.random(shots)
+        draw = np.where(u01 < P0, 0, 1)
+        bits[:, m - 1] = draw
+        Psofar = _next_psofar(cfg.get("chain", "reference"), P, P0, Psofar, draw)
+    return ["".join(str(int(b)) for b in row[ncon:]) for row in bits]

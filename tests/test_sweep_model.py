@@ -115,11 +115,33 @@ TOP = 63  # kept in act: bit 63 of eta is D_a's top bit
 U64 = (1 << 64) - 1
 
 
-def elim_loop(J, act, W, count):
+def iso_batch(J, act, W, count):
+    """The batch of CSE_ELIM_LOOP_ISO at a term's first C step: the active
+    variables whose row meets no active variable below bit 61 (its own bit
+    included) and whose bit 62 is clear, the end row and bit 63 excepted,
+    leave act together with W -= 16 each; one of them with D = 4 (bit 63)
+    ends the loop through the vanishing exit (s56 = 0).  Returns (act, W,
+    vanished)."""
+    hmask = ((act >> 32) & 0x1FFFFFFF) | 0x40000000
+    Z = 0
+    for x in range(64):
+        if ((J[x] & act & U32) | ((J[x] >> 32) & hmask)) == 0:
+            Z |= 1 << x
+    Z &= act & ~((1 << END) | (1 << TOP))
+    count["iso_batch"] += 1
+    if any((J[x] >> 63) & 1 for x in bits(Z)):
+        count["iso_vanish"] += 1
+        return act, W, True
+    count["iso_vars"] += popc(Z)
+    return act & ~Z, (W - 16 * popc(Z)) & U32, False
+
+
+def elim_loop(J, act, W, count, iso=False):
     """The hand-scheduled elimination loop (CSE_ELIM_LOOP) on packed rows
     J_x | D_x << 61, act holding the end row and bit 63; returns (s56, W):
     s56 is the index of the variable that ended the loop (END at the end
-    row)."""
+    row).  iso: CSE_ELIM_LOOP_ISO, the batch of uncoupled variables at the
+    first C step."""
     while True:
         a = low(act)
         act &= ~(1 << a)
@@ -141,6 +163,11 @@ def elim_loop(J, act, W, count):
                 return a, W
             count["C"] += 1
             W = (W + 0xFFFFFFF0) & U32
+            if iso:
+                iso = False
+                act, W, gone = iso_batch(J, act, W, count)
+                if gone:
+                    return 0, W
             continue
         count["B"] += 1                          # constraint eta . z = D_a/4
         act &= ~(1 << low(eta))
@@ -294,7 +321,7 @@ def batch_ip_gen(C, r, xj, count):
     working G rows (reset from the record's at the first one) and whose h bit
     from the working h; an elimination of i over S also takes G_b ^= G_i on
     the rows of S (row i itself becomes 0; it is retired) and h ^= G_i when
-    c = 1.  Returns the parked W word."""
+    c = 1.  The loop is CSE_ELIM_LOOP_ISO.  Returns the parked W word."""
     xj, T = table_entry(C, xj)
     J0 = C["J0"]
     B = C["actH"] & ~xj
@@ -333,7 +360,7 @@ def batch_ip_gen(C, r, xj, count):
                 count["rg_jii"] += 1
             W = restrict_step(J, S, c, W)
             count["rg_elim"] += 1
-    s56, W = elim_loop(J, act, W, count)
+    s56, W = elim_loop(J, act, W, count, iso=True)
     return W if s56 == END else 0
 
 
@@ -490,7 +517,8 @@ def test_general_restriction_branches():
     # random states' records
     for t, r, k, seed in [(12, 3, 6, 12), (30, 4, 10, 13), (20, 2, 8, 15), (16, 1, 10, 16)]:
         count += run_model(t, r, k, seed, nth=3, nj=24)
-    for kind in ("rg", "rg_elim", "rg_c", "rg_jii", "rg_empty", "rg_vanish", "A", "B", "C"):
+    for kind in ("rg", "rg_elim", "rg_c", "rg_jii", "rg_empty", "rg_vanish", "A", "B", "C", "iso_batch",
+                 "iso_vars", "iso_vanish"):
         assert count[kind] > 0, (kind, dict(count))
 
 
