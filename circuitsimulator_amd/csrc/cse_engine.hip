@@ -632,7 +632,7 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
         d.rec_off = pl.nrec;
         d.term_off = pl.nterms;
         pl.nrec += d.chi;
-        pl.nitems += d.chi * d.nchunks;
+        pl.nitems += tri_items(d.chi, d.chunk, d.nchunks);
         pl.nterms += d.chi * (d.chi + 1) / 2;
         pl.rec_off.push_back(pl.nrec);
         pl.item_off.push_back(pl.nitems);

@@ -25,6 +25,19 @@ struct DevProjector {
 //   w[69 .. 133)   J rows (symmetric; the diagonal is not used)
 #define REC_WORDS 133
 
+// Exact mode enumerates only the items that reach the diagonal: record r has
+// the chunks c >= floor(r / ch) of its j (the chunk holding j = r and the
+// ones after it), nch - floor(r / ch) items.  Block g (records g ch ..
+// g ch + ch - 1) thus has ch (nch - g) items, tri_before(g) items precede
+// it, and the last block holds the chi - (nch - 1) ch remaining records with
+// one item each.
+inline __host__ __device__ long long tri_before(long long ch, long long nch, long long g) {
+    return ch * (g * (2 * nch - g + 1) / 2);
+}
+inline __host__ __device__ long long tri_items(long long chi, long long ch, long long nch) {
+    return tri_before(ch, nch, nch - 1) + (chi - (nch - 1) * ch);
+}
+
 // Histogram bins per instance: P in [-(64 + 64), 0], 8 phases.
 #define HIST_STRIDE (129 * 8)
 // Per-wave LDS histogram: p in [-64, 0], 8 phases.

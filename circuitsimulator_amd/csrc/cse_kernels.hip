@@ -760,6 +760,21 @@ DEV int ip_phiH(const RecView &rv, u64 j, int t) {
 
 
 // item -> instance by binary search on the item prefix
+// Record r and chunk c of local item l of an exact-mode instance: block g is
+// the last with tri_before(g) <= l (the root of the quadratic, then an
+// integer correction of at most a step or two), then the item within it.
+DEV void tri_item(long long l, long long ch, long long nch, long long &r, long long &c) {
+    const double a = 2.0 * (double)nch + 1.0;
+    long long g = (long long)((a - sqrt(a * a - 8.0 * (double)l / (double)ch)) * 0.5);
+    if (g < 0) g = 0;
+    if (g > nch - 1) g = nch - 1;
+    while (g < nch - 1 && tri_before(ch, nch, g + 1) <= l) g++;
+    while (g > 0 && tri_before(ch, nch, g) > l) g--;
+    const long long off = l - tri_before(ch, nch, g), per = nch - g;
+    r = g * ch + off / per;
+    c = g + off % per;
+}
+
 DEV int find_instance(const long long *item_off, int ninst, long long item) {
     int lo = 0, hi = ninst - 1;
     while (lo < hi) {
@@ -893,8 +908,13 @@ DEV void sweep_body(const SweepArgs &A) {
         const int inst = A.ninst > 1 ? find_instance(A.item_off, A.ninst, item) : 0;
         const InstDesc &D = A.inst[inst];
         const long long local = item - A.item_off[inst];
-        const long long r = local / D.nchunks;
-        const long long c = local % D.nchunks;
+        long long r, c;
+        if (A.mode == 0) {
+            r = local / D.nchunks;
+            c = local % D.nchunks;
+        } else {
+            tri_item(local, D.chunk, D.nchunks, r, c);
+        }
         long long j0 = c * (long long)D.chunk;
         long long j1 = j0 + D.chunk;
         if (j1 > D.chi) j1 = D.chi;
