@@ -625,26 +625,40 @@ DEV int phiL_unpack(int w) {
     "s_lshr_b32 s50, %[hq], 1\n\t" \
     "s_branch L_rnext%=\n"
 
-#define PHIL_RESTRICT_ALL \
-    "L_rs%=:\n\t" \
-    "v_mov_b32 %[glo], %[g0lo]\n\t" \
-    "v_mov_b32 %[ghi], %[g0hi]\n\t" \
-    "s_mov_b64 s[48:49], %[h0]\n" \
-    "L_rdo%=:\n\t" \
+// One restriction's column, right side and pivot set: q = the lowest
+// pending column, its ballot over the working G rows, c = h_q + |col & B|
+// (bit 0 of s51), S = col & act; an empty S branches to EMPTY.
+#define PHIL_RHEAD(EMPTY) \
     "s_ff1_i32_b64 s51, s[62:63]\n\t" \
     "s_bitset0_b64 s[62:63], s51\n\t" \
     "s_lshl_b64 s[52:53], 1, s51\n\t" \
     "s_lshr_b64 s[66:67], s[48:49], s51\n\t" \
-    /* the column q over the rows: a ballot of bit q of the G rows */ \
     "v_and_b32 %[vt0], s52, %[glo]\n\t" \
     "v_and_or_b32 %[vt0], s53, %[ghi], %[vt0]\n\t" \
     "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t" \
-    /* bit 0 of s51: c = h_q + |col & B|; then S = col & act (SCC) */ \
     "s_and_b64 s[46:47], s[52:53], s[44:45]\n\t" \
     "s_bcnt1_i32_b64 s51, s[46:47]\n\t" \
     "s_xor_b32 s51, s51, s66\n\t" \
     "s_and_b64 s[64:65], s[52:53], s[58:59]\n\t" \
-    "s_cbranch_scc0 L_rempty%=\n\t" \
+    "s_cbranch_scc0 " EMPTY "%=\n\t"
+
+// Any number of restrictions (exact mode).  When a term's first restriction
+// has an empty constraint, the working rows are still the record's, and
+// every pending column that misses act is empty too and stays so (an
+// elimination of i over S only changes rows in S, and G_i has no bit q):
+// one lane-parallel pass over the record's column table (lane q: column
+// q's ballot over the pivot rows, h_q in bit 63) retires them all, or ends
+// the term at zero when one of them has right side 1.
+#define PHIL_RESTRICT_ALL \
+    "L_rs%=:\n\t" \
+    "v_mov_b32 %[glo], %[g0lo]\n\t" \
+    "v_mov_b32 %[ghi], %[g0hi]\n\t" \
+    "s_mov_b64 s[48:49], %[h0]\n\t" \
+    PHIL_RHEAD("L_rfirst0") \
+    "s_branch L_relim%=\n" \
+    "L_rdo%=:\n\t" \
+    PHIL_RHEAD("L_rempty") \
+    "L_relim%=:\n\t" \
     "s_ff1_i32_b64 s56, s[64:65]\n\t" \
     "s_bitset0_b64 s[58:59], s56\n\t" \
     "v_readlane_b32 s66, %[jlo], s56\n\t" \
@@ -685,11 +699,34 @@ DEV int phiL_unpack(int w) {
     "L_rempty%=:\n\t" \
     "s_bitcmp1_b32 s51, 0\n\t" \
     "s_cbranch_scc1 L_zero%=\n\t" \
+    "s_branch L_rnext%=\n" \
+    /* the first restriction is empty: E = the pending columns that miss \
+       act (below bit 60), zero if one of them has an odd count over B    \
+       and h_q (bit 63), else they all leave */ \
+    "L_rfirst0%=:\n\t" \
+    "s_bitcmp1_b32 s51, 0\n\t" \
+    "s_cbranch_scc1 L_zero%=\n\t" \
+    "s_and_b32 s67, s59, 0x0fffffff\n\t" \
+    "s_or_b32 s50, s45, 0x80000000\n\t" \
+    "v_and_b32 %[vt0], s58, %[ctlo]\n\t" \
+    "v_and_b32 %[vt1], s67, %[cthi]\n\t" \
+    "v_or_b32 %[vt0], %[vt0], %[vt1]\n\t" \
+    "v_cmp_eq_u32_e64 s[64:65], 0, %[vt0]\n\t" \
+    "v_and_b32 %[vt0], s44, %[ctlo]\n\t" \
+    "v_and_b32 %[vt1], s50, %[cthi]\n\t" \
+    "v_bcnt_u32_b32 %[vt0], %[vt0], 0\n\t" \
+    "v_bcnt_u32_b32 %[vt0], %[vt1], %[vt0]\n\t" \
+    "v_and_b32 %[vt0], 1, %[vt0]\n\t" \
+    "v_cmp_ne_u32_e64 s[66:67], 0, %[vt0]\n\t" \
+    "s_and_b64 s[64:65], s[64:65], s[62:63]\n\t" \
+    "s_and_b64 s[66:67], s[66:67], s[64:65]\n\t" \
+    "s_cbranch_scc1 L_zero%=\n\t" \
+    "s_andn2_b64 s[62:63], s[62:63], s[64:65]\n\t" \
     "s_branch L_rnext%=\n"
 
 template <bool ALL>
 DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, unsigned lxhi, unsigned xtlo,
-                   unsigned xthi, unsigned ttab, const PhiLConst &C, u64 g0) {
+                   unsigned xthi, unsigned ttab, const PhiLConst &C, u64 g0, u64 colt) {
     int parked;
     unsigned jlo, jhi, vt0, vt1, vt2, vt3;
     if constexpr (!ALL) {
@@ -719,7 +756,7 @@ DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, uns
             : [nb] "s"(nb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [lxlo] "v"(lxlo), [lxhi] "v"(lxhi),
               [xtlo] "v"(xtlo), [xthi] "v"(xthi), [ttab] "v"(ttab), [actR] "s"(C.actR), [actH] "s"(C.actH),
               [D2] "s"(C.D2), [nonact] "s"(C.nonact), [g0lo] "v"((unsigned)g0), [g0hi] "v"((unsigned)(g0 >> 32)),
-              [h0] "s"(C.h)
+              [h0] "s"(C.h), [ctlo] "v"((unsigned)colt), [cthi] "v"((unsigned)(colt >> 32))
             : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
               "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
     }
@@ -975,6 +1012,16 @@ DEV void sweep_body(const SweepArgs &A) {
                 C.D2 = rv.D2;
                 C.nonact = nonact;
                 C.h = rv.h;
+                // ALL: lane q holds column q's ballot over the pivot rows and
+                // h_q in bit 63, for every non-pivot column q
+                u64 colt = 0;
+                if (ALL) {
+                    for (u64 rest = nonact; rest; rest &= rest - 1) {
+                        const int q = __builtin_ctzll(rest);
+                        const u64 c = ballot((rv.G >> q) & 1ull) & rv.act;
+                        if (x == q) colt = c | (((rv.h >> q) & 1ull) << 63);
+                    }
+                }
                 if (!ALL) {
                     const u64 q0m = nonact & (0ull - nonact), q1m = nonact & ~q0m;
                     const int q0 = q0m ? __builtin_ctzll(q0m) : 0, q1 = q1m ? __builtin_ctzll(q1m) : 0;
@@ -1018,7 +1065,7 @@ DEV void sweep_body(const SweepArgs &A) {
                     phiL_tables(C, xv, xtlo, xthi, ttab);
                     const int parked = phiL_unpack(phiL_batch<ALL>((unsigned)nb, jlo0, jhi0, (unsigned)lowx,
                                                                    (unsigned)(lowx >> 32), xtlo, xthi, ttab, C,
-                                                                   rv.G));
+                                                                   rv.G, colt));
                     flush(parked, nb, jb);
                 }
             } else {

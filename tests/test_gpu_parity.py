@@ -488,3 +488,30 @@ def test_exact_item_sizes_give_identical_terms(E, chunk):
     res, terms, _ = E.batch_exact([inst], chunk=chunk, want_terms=True, nterms=nl)
     assert np.array_equal(terms, terms256)
     assert (res[0].re, res[0].im, res[0].ips) == (base[0].re, base[0].im, base[0].ips)
+
+
+def test_exact_terms_through_the_empty_restriction_batch(E):
+    # the seed-200 instance (t=11, r=4, k=4): some terms' first restriction
+    # has an empty constraint, and in one of them another column that misses
+    # the free variables has right side 1; every pair term against the oracle
+    from circuitsimulator_amd.engine import Instance, Projector
+    from circuitsimulator_amd.workloads import commuting_paulis, random_L
+    rng = np.random.default_rng(200)
+    t = int(rng.integers(10, 40))
+    r = int(rng.integers(1, t // 2 + 1))
+    k = int(rng.integers(3, 9))
+    ph, xs, zs = commuting_paulis(rng, t, r)
+    L = random_L(rng, t, k)
+    assert (t, r, k) == (11, 4, 4)
+    inst = Instance()
+    inst.P = Projector.make(ph, xs, zs, t)
+    inst.exact = 0
+    inst.k = k
+    for i, v in enumerate(L):
+        inst.L[i] = v
+    chi = 1 << k
+    nl = chi * (chi + 1) // 2
+    res, terms, _ = E.batch_exact([inst], want_terms=True, nterms=nl)
+    want = oracle_exact_terms(t, k, L, 0, ph, xs, zs)
+    assert [w[0] for w in want] == list(terms)
+    assert res[0].im == 0.0

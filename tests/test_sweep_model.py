@@ -335,6 +335,7 @@ def batch_ip_gen(C, r, xj, count):
     pend = C["nonact"] & ~xj
     if pend:
         G, h = r.G[:], r.h
+        first = True
         while pend:
             q = low(pend)
             pend &= pend - 1
@@ -347,7 +348,24 @@ def batch_ip_gen(C, r, xj, count):
                     count["rg_vanish"] += 1
                     return 0
                 count["rg_empty"] += 1
+                if first:
+                    # the first restriction is empty: every pending column
+                    # that misses act, from the record's column table (lane q:
+                    # the column's ballot over the pivot rows, h_q in bit 63)
+                    count["rg_batch"] += 1
+                    E = 0
+                    for qq in bits(pend):
+                        colt = (sum(1 << x for x in range(64) if (r.G[x] >> qq) & 1) & r.act) | (((r.h >> qq) & 1) << 63)
+                        if colt & act & ((1 << 60) - 1) == 0:
+                            E |= 1 << qq
+                            if popc(colt & (B | (1 << 63))) & 1:
+                                count["rg_batch_vanish"] += 1
+                                return 0
+                    count["rg_batch_cols"] += popc(E)
+                    pend &= ~E
+                first = False
                 continue
+            first = False
             i = low(S)
             act &= ~(1 << i)
             Gi = G[i]
@@ -518,7 +536,7 @@ def test_general_restriction_branches():
     for t, r, k, seed in [(12, 3, 6, 12), (30, 4, 10, 13), (20, 2, 8, 15), (16, 1, 10, 16)]:
         count += run_model(t, r, k, seed, nth=3, nj=24)
     for kind in ("rg", "rg_elim", "rg_c", "rg_jii", "rg_empty", "rg_vanish", "A", "B", "C", "iso_batch",
-                 "iso_vars", "iso_vanish"):
+                 "iso_vars", "iso_vanish", "rg_batch", "rg_batch_cols"):
         assert count[kind] > 0, (kind, dict(count))
 
 
@@ -550,3 +568,31 @@ def test_exponent_above_64_on_diagonal_terms():
             pmax = max(pmax, rec.k + popc(xj) + p)  # n0 + nC of the sweep
     # the case that needs the exponent field's base above 64
     assert pmax > 64
+
+
+def _seeded_exact_instance(seed):
+    """The instance a seed draws: t ~ U{10..39}, r ~ U{1..t/2}, k ~ U{3..8},
+    then the projector and L."""
+    from circuitsimulator_amd.workloads import commuting_paulis, random_L
+    rng = np.random.default_rng(seed)
+    t = int(rng.integers(10, 40))
+    r = int(rng.integers(1, t // 2 + 1))
+    k = int(rng.integers(3, 9))
+    ph, xs, zs = commuting_paulis(rng, t, r)
+    return t, r, k, ph, xs, zs, random_L(rng, t, k)
+
+
+def test_empty_first_restriction_batch_can_vanish():
+    """Record 7 against phi_0 of the seed-200 instance: its first restriction
+    has an empty constraint, and another column that misses the free
+    variables has right side 1, so the batch ends the term at zero."""
+    t, r, k, ph, xs, zs, L = _seeded_exact_instance(200)
+    assert (t, r, k) == (11, 4, 4)
+    th = orc.prepL(7, t, L)
+    assert orc.project(th, orc.make_projector(ph, xs, zs, t)) >= 0
+    rec = to_record(th)
+    count = collections.Counter()
+    got = decode(unpack(batch_ip_gen(batch_consts(rec, t), rec, xj_of(0, L, k), count)))
+    e, p, m = orc.inner(th, orc.prepL(0, t, L))
+    assert got == ((e, p, m % 8) if e else (0, 0, 0)) == (0, 0, 0)
+    assert count["rg_batch_vanish"] == 1
