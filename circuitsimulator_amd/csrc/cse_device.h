@@ -49,11 +49,43 @@ DEV u64 shfl_xor64(u64 v, int m) {
     int hi = __shfl_xor((int)(unsigned)(v >> 32), m, 64);
     return ((u64)(unsigned)hi << 32) | (unsigned)lo;
 }
-// XOR of v over all lanes (butterfly); returned as a wave-uniform value
+// Cross-lane moves within the rows of 16 lanes (DPP, in registers):
+// quad_perm [1,0,3,2] (0xB1) and [2,3,0,1] (0x4E), row_ror 4 (0x124) and 8
+// (0x128).  Every source lane is inside the row, and the callers run with
+// all 64 lanes active.
+template <int CTRL> DEV unsigned dpp_mov(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+// XOR of a 32-bit value over each row: the two quad exchanges give every lane
+// its quad's XOR, the two rotations fold the row's four quads
+DEV unsigned row_xor32(unsigned v) {
+    v ^= dpp_mov<0xB1>(v);
+    v ^= dpp_mov<0x4E>(v);
+    v ^= dpp_mov<0x124>(v);
+    v ^= dpp_mov<0x128>(v);
+    return v;
+}
+DEV unsigned row_or32(unsigned v) {
+    v |= dpp_mov<0xB1>(v);
+    v |= dpp_mov<0x4E>(v);
+    v |= dpp_mov<0x124>(v);
+    v |= dpp_mov<0x128>(v);
+    return v;
+}
+// XOR of v over all lanes, wave-uniform: rows reduced with DPP, the four
+// rows' results (lanes 0, 16, 32, 48) combined in scalar registers
 DEV u64 wave_xor(u64 v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v ^= shfl_xor64(v, off);
-    return uniform64(v);
+    const int lo = (int)row_xor32((unsigned)v), hi = (int)row_xor32((unsigned)(v >> 32));
+    const unsigned l = (unsigned)(rl32(lo, 0) ^ rl32(lo, 16) ^ rl32(lo, 32) ^ rl32(lo, 48));
+    const unsigned h = (unsigned)(rl32(hi, 0) ^ rl32(hi, 16) ^ rl32(hi, 32) ^ rl32(hi, 48));
+    return ((u64)h << 32) | l;
+}
+// OR of v over all lanes, wave-uniform (as wave_xor)
+DEV u64 wave_or(u64 v) {
+    const int lo = (int)row_or32((unsigned)v), hi = (int)row_or32((unsigned)(v >> 32));
+    const unsigned l = (unsigned)(rl32(lo, 0) | rl32(lo, 16) | rl32(lo, 32) | rl32(lo, 48));
+    const unsigned h = (unsigned)(rl32(hi, 0) | rl32(hi, 16) | rl32(hi, 32) | rl32(hi, 48));
+    return ((u64)h << 32) | l;
 }
 
 // ------------------------------------------------------------------ Philox

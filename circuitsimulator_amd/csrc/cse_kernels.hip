@@ -233,11 +233,6 @@ DEV int rs_project(RState &s, const DevProjector &P) {
     return sexp;
 }
 
-DEV u64 wave_or(u64 v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v |= shfl_xor64(v, off);
-    return uniform64(v);
-}
 
 // Reduced row echelon basis of the active rows (basis changes of the form,
 // so the state is unchanged), then rows and columns of J and the D planes
