@@ -231,16 +231,20 @@ DEV void rs_extend(RState &s, u64 xi) {
 DEV int rs_measure_pauli(RState &s, int m, u64 zeta, u64 xi) {
     const int x = lane_id();
     const bool act = x < s.k;
-    u64 vxi = ballot(act && par64(s.Gb & xi));
+    // coordinates of xi in the basis G (G Gbar^T = I over all n rows): xi is
+    // in span(G_0 .. G_{k-1}) (xi' = sum of the rows in vxi equals xi)
+    // exactly when xi . Gbar_b = 0 for every b in [k, n)
+    const bool pxi = par64(s.Gb & xi);
+    u64 vxi = ballot(act && pxi);
+    const bool in_span = ballot(!act && x < s.n && pxi) == 0;
     u64 vze = ballot(act && par64(s.G & zeta));
-    u64 xip = wave_xor(((vxi >> x) & 1) ? s.G : 0ull);
     bool inx = (vxi >> x) & 1;
     int w = 2 * m + 4 * par64(zeta & s.h);
     w += 2 * __popcll(ballot(inx && (s.d & 2))) + 4 * __popcll(ballot(inx && (s.d & 4)));
     w += 4 * __popcll(ballot(inx && par64(s.J & vxi & lowmask(x))));
     w &= 7;
     u64 eta = ballot(x < s.n && par64(s.J & vxi)) ^ vze;
-    if (xi == xip) {
+    if (in_span) {
         if (w == 0 || w == 4) {
             u64 gamma = wave_xor((act && ((eta >> x) & 1)) ? s.Gb : 0ull);
             int alpha = ((w >> 2) + __popcll(gamma & s.h)) & 1;
