@@ -175,31 +175,6 @@ DEV void rs_prepL(RState &s, u64 i, int t, const uint64_t *L, int k) {
         if (!((xv >> q) & 1ull)) rs_shrink(s, bit(q), 0, false);
 }
 
-// The same state as rs_prepL, K = span{e_q : q in supp(x_i)}, h = 0, no
-// phase, built directly for records (whose later reduction to echelon form
-// makes the row order irrelevant): rows 0..k-1 are the unit vectors of the
-// support in increasing q, rows k..t-1 the others, so G is a permutation
-// and Gb = G.  Lane q sends q to the row r(q) it lands in.
-DEV void rs_prepL_direct(RState &s, u64 i, int t, const uint64_t *L, int k) {
-    const int x = lane_id();
-    u64 xv = 0;
-    for (int j = 0; j < k; j++)
-        if ((i >> (k - 1 - j)) & 1ull) xv ^= L[j];
-    const u64 below = lowmask(x);
-    const int kk = __popcll(xv);
-    int r = x;
-    if (x < t) r = ((xv >> x) & 1ull) ? __popcll(xv & below) : kk + __popcll(~xv & lowmask(t) & below);
-    const int q = __builtin_amdgcn_ds_permute(r << 2, x);
-    s.G = x < t ? bit(q) : 0ull;
-    s.Gb = s.G;
-    s.J = 0;
-    s.d = 0;
-    s.n = t;
-    s.k = kk;
-    s.Q = 0;
-    s.h = 0;
-}
-
 // stateprep.c:36-81 in the reference row bookkeeping
 DEV void rs_prepH(RState &s, u64 i, int t) {
     const int x = lane_id();
@@ -310,6 +285,91 @@ k_random_states(int n, const uint64_t *__restrict__ thr, uint64_t seed, long lon
     if (sexp && lane_id() == 0) sexp[wave] = se;
 }
 
+// The record of Pi prepL(x) built directly in reduced row echelon form,
+// rows at their pivot lanes: the product state (|+> on supp(x), |0>
+// elsewhere) has act = supp(x), G_a = e_a, h = 0 and no form.  Each generator
+// i^m Z(zeta) X(xi) (the cases of measurePauli, stabilizer.c:827-959) works
+// on the coordinates c = xi & act of xi in that basis and the residual
+// rho = xi + sum_{a in c} G_a (zero exactly when xi is in the span):
+//   w = 2m + 4 zeta.h + sum_{a in c} D_a + 4 #edges(J inside c) (mod 8),
+//   eta_b = (J c)_b + zeta.G_b on act (J_bb = [D_b in {2,6}]);
+//  rho = 0, w in {0,4}: the constraint eta . z = w/4 -- nothing (eta empty,
+//    w = 0), zero (eta empty, w = 4), else an elimination of min(eta);
+//  rho = 0, w in {2,6}: a factor 2^-1/2 w^{+-1}: Q += s, D -= 2s and J ^= eta
+//    eta^T on eta (s = 1 for w = 2, -1 for 6);
+//  rho != 0: a new variable at p = min(rho) with G_p = rho, D_p =
+//    w + 4 zeta.rho, J row and column eta, then the basis change that clears
+//    column p from the other rows.
+// Returns the number of 2^-1/2 factors, -1 if the projection vanishes.
+DEV int rs_project_rref(AState &s, const DevProjector &P) {
+    const int x = lane_id();
+    int sexp = 0;
+    for (int g = 0; g < P.nstabs; g++) {
+        const int m = P.phase[g] & 3;
+        const u64 zeta = P.zs[g], xi = P.xs[g];
+        const u64 c = xi & s.act;
+        const bool inA = lane_in(s.act), inC = lane_in(c);
+        const u64 rho = xi ^ wave_xor(inC ? s.G : 0ull);
+        const u64 vze = ballot(inA && par64(s.G & zeta));
+        const u64 d1 = ballot(inC && (s.d & 2)), d2 = ballot(inC && (s.d & 4));
+        const u64 edg = ballot(inC && par64(s.J & c & lowmask(x)));
+        const int w = (2 * m + 4 * par64(zeta & s.h) + 2 * __popcll(d1) + 4 * __popcll(d2) + 4 * __popcll(edg)) & 7;
+        const u64 jb = (s.J & ~bit(x)) | ((u64)((s.d >> 1) & 1) << x);
+        const u64 eta = ballot(inA && par64(jb & c)) ^ vze;
+        if (rho == 0 && (w & 2) == 0) {
+            if (eta == 0) {
+                if (w) return -1;
+                continue;
+            }
+            const int i = __builtin_ctzll(eta);
+            as_eliminate<false>(s, i, eta & ~bit(i), w >> 2, true);
+        } else if (rho == 0) {
+            const int sg = w == 2 ? 1 : -1;
+            s.Q = (s.Q + sg) & 7;
+            if (lane_in(eta)) {
+                s.d = (s.d - 2 * sg) & 7;
+                s.J ^= eta;
+            }
+        } else {
+            const int p = __builtin_ctzll(rho);
+            const int dp = (w + 4 * par64(zeta & rho)) & 7;
+            if (inA) s.J = (s.J & ~bit(p)) | (((eta >> x) & 1ull) << p);
+            if (x == p) {
+                s.G = rho;
+                s.d = dp;
+                s.J = eta | ((u64)((dp >> 1) & 1) << p);
+            }
+            s.act |= bit(p);
+            const u64 T = ballot(lane_in(s.act) && x != p && ((s.G >> p) & 1ull));
+            if (T) {
+                int dummy;
+                as_basis_change<false>(s, p, T, true, &dummy);
+            }
+        }
+        sexp++;
+    }
+    return sexp;
+}
+
+// The record of a state already in reduced row echelon form at its pivot
+// lanes (the layout to_record produces): J rows restricted to act, the rows
+// of non-pivot lanes zero.
+DEV void store_rref(const AState &s, int n, int sexp, uint64_t *rec) {
+    const int x = lane_id();
+    const bool a = lane_in(s.act);
+    const u64 D1 = ballot(a && (s.d & 2)), D2 = ballot(a && (s.d & 4));
+    if (x == 0) {
+        rec[0] = (uint64_t)__popcll(s.act) | ((uint64_t)n << 8) | ((uint64_t)(s.Q & 7) << 16) |
+                 ((uint64_t)sexp << 24);
+        rec[1] = s.h;
+        rec[2] = s.act;
+        rec[3] = D1;
+        rec[4] = D2;
+    }
+    rec[5 + x] = a ? s.G : 0ull;
+    rec[69 + x] = a ? (s.J & s.act) : 0ull;
+}
+
 // Records of projected states, one wave per record g (global index over all
 // instances; instance found from the record prefix).
 // mode 0: theta_s = random stabilizer state s = first + local index of the
@@ -349,7 +409,22 @@ k_records(int mode, int ninst, const long long *__restrict__ rec_off, const Inst
     } else if (D.exact) {
         rs_prepH(s, (u64)local, n);
     } else {
-        rs_prepL_direct(s, (u64)local, n, D.L, D.k);
+        // |L> records directly in reduced row echelon form
+        u64 xv = 0;
+        for (int j = 0; j < D.k; j++)
+            if (((u64)local >> (D.k - 1 - j)) & 1ull) xv ^= D.L[j];
+        AState a;
+        a.act = xv & lowmask(n);
+        a.G = lane_in(a.act) ? bit(lane_id()) : 0ull;
+        a.J = 0;
+        a.d = 0;
+        a.h = 0;
+        a.Q = 0;
+        const int se = rs_project_rref(a, projs[in]);
+        if (lane_id() == 0) sexp_out[g] = se;
+        if (se < 0) Rec::store_vanished(rec, n);
+        else store_rref(a, n, se, rec);
+        return;
     }
     const int se = rs_project(s, projs[in]);
     if (lane_id() == 0) sexp_out[g] = se;

@@ -596,3 +596,150 @@ def test_empty_first_restriction_batch_can_vanish():
     e, p, m = orc.inner(th, orc.prepL(0, t, L))
     assert got == ((e, p, m % 8) if e else (0, 0, 0)) == (0, 0, 0)
     assert count["rg_batch_vanish"] == 1
+
+
+def project_rref(t, xv, gens, count):
+    """The record of Pi prepL(x) built directly in reduced row echelon form
+    (rows at their pivot lanes), as k_records does for exact-mode records:
+    the product state has act = supp(x) and G_a = e_a; each generator
+    i^m Z(zeta) X(xi) (the cases of measurePauli, stabilizer.c:827-959) then
+    works on the coordinates c = xi & act of xi in that basis.  Returns (the
+    record as a Form, sexp), or (None, -1) when the projection vanishes."""
+    s = Form()
+    s.act = xv & ((1 << t) - 1)
+    for a in bits(s.act):
+        s.G[a] = 1 << a
+    sexp = 0
+    for m, zeta, xi in gens:
+        act = s.act
+        c = xi & act
+        rho = xi
+        for a in bits(c):
+            rho ^= s.G[a]
+        vze = sum(1 << a for a in bits(act) if popc(s.G[a] & zeta) & 1)
+        w = 2 * m + 4 * (popc(zeta & s.h) & 1) + sum(s.d[a] for a in bits(c))
+        w += 4 * sum(1 for a in bits(c) for b in bits(c) if a < b and (s.J[a] >> b) & 1)
+        w %= 8
+        eta = 0
+        for b in bits(act):
+            jb = (s.J[b] & ~(1 << b)) | (((s.d[b] >> 1) & 1) << b)
+            if (popc(jb & c) + ((vze >> b) & 1)) & 1:
+                eta |= 1 << b
+        if rho == 0 and w in (0, 4):
+            beta = w >> 2
+            if eta == 0:
+                count["pj_same" if not beta else "pj_vanish"] += 1
+                if beta:
+                    return None, -1
+                continue
+            count["pj_shrink"] += 1
+            i = low(eta)
+            M, di = basis_change(s, i, eta & ~(1 << i), True)
+            if beta:
+                s.Q = (s.Q + di) % 8
+                for x in bits(M):
+                    s.d[x] ^= 4
+                s.h ^= s.G[i]
+            s.act &= ~(1 << i)
+        elif rho == 0:
+            count["pj_phase"] += 1
+            sigma = 1 if w == 2 else -1
+            s.Q = (s.Q + sigma) % 8
+            for b in bits(eta):
+                s.d[b] = (s.d[b] - 2 * sigma) % 8
+                s.J[b] ^= eta
+        else:
+            count["pj_extend"] += 1
+            p = low(rho)
+            Dp = (w + 4 * (popc(zeta & rho) & 1)) % 8
+            for b in bits(act):
+                s.J[b] = (s.J[b] & ~(1 << p)) | (((eta >> b) & 1) << p)
+            s.G[p], s.d[p] = rho, Dp
+            s.J[p] = eta | (((Dp >> 1) & 1) << p)
+            s.act |= 1 << p
+            T = sum(1 << b for b in bits(s.act) if b != p and (s.G[b] >> p) & 1)
+            if T:
+                basis_change(s, p, T, True)
+        sexp += 1
+    r = Form()
+    r.act, r.h, r.Q, r.k = s.act, s.h, s.Q % 8, popc(s.act)
+    for a in bits(s.act):
+        r.G[a] = s.G[a]
+        r.d[a] = s.d[a] % 8
+        r.J[a] = s.J[a] & s.act
+    return r, sexp
+
+
+def _gens_of(ph, xs, zs):
+    return [(int(p) & 3, int(z), int(x)) for p, x, z in zip(ph, xs, zs)]
+
+
+@pytest.mark.parametrize("t,r,k,seed", [(24, 12, 6, 41), (40, 18, 5, 42), (60, 20, 4, 43), (9, 4, 4, 44),
+                                        (48, 1, 6, 45), (12, 6, 3, 46)])
+def test_direct_rref_projection_matches_oracle(t, r, k, seed):
+    # the directly built record: same projection exponent as the oracle's
+    # measurePauli sequence, and every sampled term through the exact-mode
+    # record path equal to the oracle's inner product
+    from circuitsimulator_amd.workloads import commuting_paulis, random_L
+    rng = np.random.default_rng(seed)
+    ph, xs, zs = commuting_paulis(rng, t, r)
+    L = random_L(rng, t, k)
+    Po = orc.make_projector(ph, xs, zs, t)
+    gens = _gens_of(ph, xs, zs)
+    count = collections.Counter()
+    chi = 1 << k
+    js = sorted(set(int(v) for v in rng.integers(0, chi, size=10)))
+    for i in range(chi):
+        th = orc.prepL(i, t, L)
+        e = orc.project(th, Po)
+        rec, se = project_rref(t, xj_of(i, L, k), gens, count)
+        assert se == e, (i, se, e)
+        if e < 0:
+            continue
+        C = batch_consts(rec, t)
+        for j in js:
+            ee, p, m = orc.inner(th, orc.prepL(j, t, L))
+            want = (ee, p, m % 8) if ee else (0, 0, 0)
+            got = decode(unpack(batch_ip_gen(C, rec, xj_of(j, L, k), count)))
+            assert got == want, (i, j, got, want)
+    assert count["pj_extend"] + count["pj_phase"] + count["pj_shrink"] > 0
+
+
+def test_direct_rref_projection_branches():
+    from circuitsimulator_amd.workloads import commuting_paulis, random_L
+    count = collections.Counter()
+    for seed in range(50, 62):
+        rng = np.random.default_rng(seed)
+        t = int(rng.integers(6, 30))
+        r = int(rng.integers(1, t // 2 + 1))
+        k = int(rng.integers(2, 6))
+        ph, xs, zs = commuting_paulis(rng, t, r)
+        L = random_L(rng, t, k)
+        Po = orc.make_projector(ph, xs, zs, t)
+        for i in range(1 << k):
+            rec, se = project_rref(t, xj_of(i, L, k), _gens_of(ph, xs, zs), count)
+            assert se == orc.project(orc.prepL(i, t, L), Po)
+    # Z-type generators (all commute; phase 0 or 2) on qubits outside supp(x)
+    # leave the product state unchanged or annihilate it
+    for seed in range(62, 70):
+        rng = np.random.default_rng(seed)
+        t, k = 12, 3
+        L = [int(v) for v in rng.integers(1, 1 << t, size=k)]
+        zs = [int(v) for v in rng.integers(1, 1 << t, size=4)] + [1 << int(q) for q in rng.integers(0, t, size=3)]
+        ph = [2 * int(b) for b in rng.integers(0, 2, size=len(zs))]
+        xs = [0] * len(zs)
+        Po = orc.make_projector(ph, xs, zs, t)
+        for i in range(1 << k):
+            th = orc.prepL(i, t, L)
+            rec, se = project_rref(t, xj_of(i, L, k), _gens_of(ph, xs, zs), count)
+            e = orc.project(th, Po)
+            assert se == e, (seed, i)
+            if e < 0:
+                continue
+            C = batch_consts(rec, t)
+            for j in range(1 << k):
+                ee, pp, mm = orc.inner(th, orc.prepL(j, t, L))
+                want = (ee, pp, mm % 8) if ee else (0, 0, 0)
+                assert decode(unpack(batch_ip_gen(C, rec, xj_of(j, L, k), count))) == want, (seed, i, j)
+    for kind in ("pj_extend", "pj_phase", "pj_shrink", "pj_same", "pj_vanish"):
+        assert count[kind] > 0, (kind, dict(count))
