@@ -515,3 +515,26 @@ def test_exact_terms_through_the_empty_restriction_batch(E):
     want = oracle_exact_terms(t, k, L, 0, ph, xs, zs)
     assert [w[0] for w in want] == list(terms)
     assert res[0].im == 0.0
+
+
+@pytest.mark.parametrize("t", [61, 64])
+def test_exact_unpacked_layout_all_terms(E, t):
+    # exact-mode |L> records (built directly in echelon form) past the packed
+    # layout's limit run the unpacked sweep; every pair term against the oracle
+    from circuitsimulator_amd.engine import Instance, Projector
+    rng = np.random.default_rng(6100 + t)
+    r, k = 9, 4
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    L = [orc.rand_bits(rng, t) for _ in range(k)]
+    inst = Instance()
+    inst.P = Projector.make(ph, xs, zs, t)
+    inst.exact = 0
+    inst.k = k
+    for i, v in enumerate(L):
+        inst.L[i] = v
+    chi = 1 << k
+    nl = chi * (chi + 1) // 2
+    res, terms, _ = E.batch_exact([inst], chunk=5, want_terms=True, nterms=nl)
+    want = oracle_exact_terms(t, k, L, 0, ph, xs, zs)
+    assert [w[0] for w in want] == list(terms)
+    assert res[0].im == 0.0
