@@ -743,3 +743,112 @@ def test_direct_rref_projection_branches():
                 assert decode(unpack(batch_ip_gen(C, rec, xj_of(j, L, k), count))) == want, (seed, i, j)
     for kind in ("pj_extend", "pj_phase", "pj_shrink", "pj_same", "pj_vanish"):
         assert count[kind] > 0, (kind, dict(count))
+
+
+def batch_ip_H(r, j, t, count, iso=True):
+    """One |H^t> term of phiH_batch: <prepH(j) | theta'> from the record r
+    (stateprep.c:36-81: pair q = (2q, 2q+1) is controlled by bit size-1-q of
+    j; bit 1 makes it |00> + |11>, the constraint x_2q = x_2q+1, bit 0 the
+    four-term state, the coupling 4 x_2q x_2q+1; for odd t the last qubit is
+    |0> on bit 1 and |+> on bit 0).  The constraints come first, set bits of
+    j from the lowest (restrictions on working G rows and h, as in
+    batch_ip_gen), then the couplings, clear pair bits from the lowest (on
+    the working rows: Q += 4 h_u h_v, D += 4 on (cu & hv) ^ (cv & hu) ^
+    (cu & cv), J ^= cv on cu and cu on cv, cu and cv the column ballots of u
+    and v), then the loop.  W starts at 16 (k + t - #constraints - n0) + 1
+    with Q in m.  Returns the parked W word."""
+    size = (t + 1) // 2
+    odd = t & 1
+    J = packed_rows(r)
+    G, h = r.G[:], r.h
+    act = r.act | (1 << END) | (1 << TOP)
+    n0 = popc(r.act)
+    jm = j & ((1 << size) - 1)
+    W = (((r.Q & 7) << 29) + 16 * (r.k + t - popc(jm) - n0) + 1) & U32
+    for b in bits(jm):
+        q = size - 1 - b
+        xi = (1 << (t - 1)) if (odd and q == size - 1) else (3 << (2 * q))
+        count["h_restrict"] += 1
+        col = sum(1 << x for x in range(64) if popc(G[x] & xi) & 1)
+        S = col & act
+        c = popc(h & xi) & 1
+        if not S:
+            if c:
+                count["h_vanish"] += 1
+                return 0
+            continue
+        i = low(S)
+        act &= ~(1 << i)
+        Gi = G[i]
+        for x in bits(S):
+            G[x] ^= Gi
+        if c:
+            h ^= Gi
+        W = restrict_step(J, S, c, W)
+    pairs = ((1 << size) - 1) & ~(1 if odd else 0)
+    for b in bits(~jm & pairs):
+        q = size - 1 - b
+        u, v = 2 * q, 2 * q + 1
+        count["h_couple"] += 1
+        cu = sum(1 << x for x in range(64) if (G[x] >> u) & 1) & act
+        cv = sum(1 << x for x in range(64) if (G[x] >> v) & 1) & act
+        hu, hv = (h >> u) & 1, (h >> v) & 1
+        if hu & hv:
+            W = (W + 0x80000000) & U32
+        lin = (cv if hu else 0) ^ (cu if hv else 0) ^ (cu & cv)
+        for x in bits(lin):
+            J[x] ^= 1 << 63
+        for x in bits(cu):
+            J[x] ^= cv
+        for x in bits(cv):
+            J[x] ^= cu
+    s56, W = elim_loop(J, act, W, count, iso=iso)
+    return W if s56 == END else 0
+
+
+@pytest.mark.parametrize("t,r,seed", [(6, 3, 71), (7, 2, 72), (12, 5, 73), (13, 6, 74), (20, 6, 75), (9, 4, 76)])
+def test_phiH_terms_match_oracle(t, r, seed):
+    # exact-mode records (Pi prepH(i)) and random-state records against every
+    # prepH(j): the |H^t> batch form equals the oracle's inner product
+    from circuitsimulator_amd.workloads import commuting_paulis
+    rng = np.random.default_rng(seed)
+    ph, xs, zs = commuting_paulis(rng, t, r)
+    Po = orc.make_projector(ph, xs, zs, t)
+    chi = 1 << ((t + 1) // 2)
+    js = list(range(chi)) if chi <= 64 else [int(v) for v in rng.integers(0, chi, size=40)]
+    phis = {j: orc.prepH(j, t) for j in js}
+    count = collections.Counter()
+    thr = orc.thresholds(t)
+    states = [orc.prepH(int(i), t) for i in rng.integers(0, chi, size=6)]
+    states += [orc.random_state(t, seed, s, thr) for s in range(6)]
+    for th in states:
+        if orc.project(th, Po) < 0:
+            continue
+        rec = to_record(th)
+        for j in js:
+            e, p, m = orc.inner(th, phis[j])
+            want = (e, p, m % 8) if e else (0, 0, 0)
+            for iso in (True, False):
+                got = decode(unpack(batch_ip_H(rec, j, t, count, iso=iso)))
+                assert got == want, (t, j, iso, got, want)
+    assert count["h_restrict"] > 0 and count["h_couple"] > 0
+
+
+def test_phiH_branches():
+    from circuitsimulator_amd.workloads import commuting_paulis
+    count = collections.Counter()
+    for t, r, seed in [(6, 3, 71), (7, 2, 72), (12, 5, 73), (13, 6, 74), (20, 6, 75), (9, 4, 76)]:
+        rng = np.random.default_rng(seed)
+        ph, xs, zs = commuting_paulis(rng, t, r)
+        Po = orc.make_projector(ph, xs, zs, t)
+        thr = orc.thresholds(t)
+        for s in range(6):
+            th = orc.random_state(t, seed, s, thr)
+            if orc.project(th, Po) < 0:
+                continue
+            rec = to_record(th)
+            for j in range(min(64, 1 << ((t + 1) // 2))):
+                e, p, m = orc.inner(th, orc.prepH(j, t))
+                assert decode(unpack(batch_ip_H(rec, j, t, count))) == ((e, p, m % 8) if e else (0, 0, 0))
+    for kind in ("h_restrict", "h_couple", "h_vanish", "A", "B", "C", "iso_batch"):
+        assert count[kind] > 0, (kind, dict(count))
