@@ -134,6 +134,38 @@ def exact_pairs_config(args):
     }
 
 
+def exact_h_config(args):
+    """The exact projector with the |H^t> decomposition (probability()'s
+    default, innerprod.c:148-261 with prepH, stateprep.c:36-81): t=30, r=10,
+    chi = 2^15, so 536,887,296 pair terms, each one inner product
+    <prepH(j)|Pi prepH(i)>.  value = pair terms/s of the whole call."""
+    from circuitsimulator_amd import Engine
+    from circuitsimulator_amd.workloads import sampled_config
+    t, r = 30, 10
+    P, _, _ = sampled_config(t, r, 1, seed=1030)
+    chi = 1 << ((t + 1) // 2)
+    nterms = chi * (chi + 1) // 2
+    with Engine(0) as E:
+        for _ in range(args.warmup):
+            E.exact_projector(P, [], 0, 1)
+        t0 = time.perf_counter()
+        kernel_ms = 0.0
+        for _ in range(args.steps):
+            res, _ = E.exact_projector(P, [], 0, 1)
+            kernel_ms += res.kernel_ms
+        dt = time.perf_counter() - t0
+    return {
+        "metric": "exact projector pair terms/s with the |H^t> decomposition, t=30 r=10",
+        "value": nterms * args.steps / dt, "unit": "inner products/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1000.0 / args.steps, "higher_is_better": True,
+        "dtype": "int64", "data": "synthetic (seeded commuting Pauli group)",
+        "config": {"workload": f"exact projector, |H^t> decomposition, t={t} r={r}, {nterms} pair terms",
+                   "t": t, "r": r, "chi": chi, "seed": 1030},
+        "kernel_ip_per_s": nterms * args.steps / (kernel_ms / 1000.0),
+        "result": {"value_int64_exact": res.value, "re": res.re, "im": res.im},
+    }
+
+
 def batch_config(args):
     """C3: 4096 uneven exact-mode instances (t ~ U{8..48}, r ~ U{1..t/2},
     k ~ U{2..14}), every pair term of every instance in one persistent
@@ -234,8 +266,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--plain-pairs", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5"], default="c4",
-                    help="c4: the headline norm estimation (default); c1, c2, c3, c5: the other configurations")
+    ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5", "h30"], default="c4",
+                    help="c4: the headline norm estimation (default); c1, c2, c3, c5: the other configurations; "
+                         "h30: the exact projector with the |H^t> decomposition at t=30")
     ap.add_argument("--shots", type=int, default=1024, help="c5: number of shots")
     ap.add_argument("--instances", type=int, default=4096, help="c3: number of instances")
     args = ap.parse_args()
@@ -247,7 +280,7 @@ def main():
 
     if args.config != "c4":
         out = {"c1": circuit_config, "c2": exact_pairs_config, "c3": batch_config,
-               "c5": shots_config}[args.config](args)
+               "c5": shots_config, "h30": exact_h_config}[args.config](args)
         print(json.dumps(out), flush=True)
         return
 
