@@ -15,12 +15,15 @@ tables of x_j and the term-independent part of the W word m * 2^29 |
 16 np + 1, fixation with one edge ballot, restrictions as constrained steps
 with the second column updated in scalar registers, the loop, W parked and
 unpacked later), its exact-mode form (every non-pivot column outside
-supp(x_j) a restriction, columns from working G rows), and the fallback of
-records with more non-pivot columns (restrictions in C++, then the same
-loop).  Every term
-is checked against the oracle's innerProduct (libcirc/stabilizer/
-stabilizer.c:589-667), and the step kinds are counted so that every branch
-is known to be exercised.
+supp(x_j) a restriction, columns from working G rows, the batch of empty
+first restrictions, the loop's batch of uncoupled variables), the |H^t>
+block phiH_batch (pair constraints and couplings), the fallback of records
+with more non-pivot columns (restrictions in C++, then the same loop), and
+the exact-mode record built directly in echelon form (rs_project_rref).
+Every term is checked against the oracle's innerProduct (libcirc/
+stabilizer/stabilizer.c:589-667), every directly built record against the
+oracle's projection, and the step kinds are counted so that every branch is
+known to be exercised.
 """
 import collections
 
