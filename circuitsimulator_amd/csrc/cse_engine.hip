@@ -275,14 +275,11 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     A.hist_stride = HIST_STRIDE;
     A.ips = dp<unsigned long long>(ctx->ips);
     A.err = dp<int>(ctx->errflag);
-    bool packed = true, hdec = false;
-    for (const InstDesc &d : pl.inst) {
-        packed = packed && d.n <= CSE_END_ROW;
-        hdec = hdec || d.exact;
-    }
-    // exact mode and |H^t> terms take k_sweep_packed_all (seven waves per
-    // SIMD); sampled |L> launches k_sweep_packed (eight)
-    const bool all = packed && (mode == 1 || hdec);
+    bool packed = true;
+    for (const InstDesc &d : pl.inst) packed = packed && d.n <= CSE_END_ROW;
+    // exact mode takes k_sweep_packed_all (seven waves per SIMD); sampled
+    // launches, |L> and |H^t> terms alike, take k_sweep_packed (eight)
+    const bool all = packed && mode == 1;
     const int grid = sweep_grid(ctx, pl.nitems, all ? 7 : ctx->blocks_per_cu);
     if (all) hipLaunchKernelGGL(k_sweep_packed_all, dim3(grid), dim3(256), 0, ctx->stream, A);
     else if (packed) hipLaunchKernelGGL(k_sweep_packed, dim3(grid), dim3(256), 0, ctx->stream, A);

@@ -14,7 +14,8 @@
 //                    accumulates exactly (int64) plus a Kahan-FP32 shadow.
 //   k_sweep_packed   the same with D packed into the J rows (n <= 60); the
 //                    |L> terms of records with at most two non-pivot columns
-//                    run 64 at a time in one asm block (phiL_batch)
+//                    run 64 at a time in one asm block (phiL_batch), every
+//                    |H^t> term in phiH_batch
 //   k_sweep_packed_all  the packed sweep of exact mode: every |L> term in
 //                    the asm block, any number of restrictions
 //   k_lspace_hist    histogram of |x| over the row space of L (decompose's
@@ -849,158 +850,175 @@ DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, uns
 //  * the couplings, clear pair bits from the lowest: cu, cv the ballots of
 //    bits u, v of the G rows (on act), Q += 4 h_u h_v, D += 4 on
 //    (cv & hu) ^ (cu & hv) ^ (cu & cv), J ^= cv on cu and J ^= cu on cv;
-//  * the loop with the batch of uncoupled variables.
+//  * the loop: with the batch of uncoupled variables in the exact-mode
+//    kernel (ALL), plain in the sampled one.
 // Registers as phiL_batch, with s46 j's pair bits (then cv), s50 u = 2q, s51
 // the pair's bit, then c, s[52:53] the pair's mask, then the column or cu,
 // s62 the pending pair bits.
 // a wave-uniform value in a scalar register
 DEV unsigned uni32(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
 
+#define PHIH_BODY(LOOP) \
+    "v_mov_b32 %[parked], 0\n\t" \
+    "v_mov_b32 v40, 0xc0000000\n\t" \
+    "s_mov_b32 s61, m0\n\t" \
+    "s_mov_b32 m0, 0\n" \
+    "L_ip%=:\n\t" \
+    "s_add_u32 s46, %[jb], m0\n\t" \
+    "s_and_b32 s46, s46, %[smask]\n\t" \
+    "s_bcnt1_i32_b32 s47, s46\n\t" \
+    "s_lshl_b32 s47, s47, 4\n\t" \
+    "s_sub_u32 s60, %[wbase], s47\n\t" \
+    "v_mov_b32 %[jlo], %[jlo0]\n\t" \
+    "v_mov_b32 %[jhi], %[jhi0]\n\t" \
+    "v_mov_b32 %[glo], %[g0lo]\n\t" \
+    "v_mov_b32 %[ghi], %[g0hi]\n\t" \
+    "s_mov_b64 s[48:49], %[h0]\n\t" \
+    "s_mov_b64 s[58:59], %[actR]\n\t" \
+    "s_mov_b32 s62, s46\n" \
+    /* constraints */ \
+    "L_hr%=:\n\t" \
+    "s_cmp_eq_u32 s62, 0\n\t" \
+    "s_cbranch_scc1 L_hc%=\n\t" \
+    "s_ff1_i32_b32 s51, s62\n\t" \
+    "s_bitset0_b32 s62, s51\n\t" \
+    "s_sub_u32 s50, %[sm1], s51\n\t" \
+    "s_lshl_b32 s50, s50, 1\n\t" \
+    "s_lshl_b64 s[52:53], 3, s50\n\t" \
+    "s_and_b64 s[66:67], s[48:49], s[52:53]\n\t" \
+    "s_bcnt1_i32_b64 s51, s[66:67]\n\t" \
+    "v_and_b32 %[vt0], s52, %[glo]\n\t" \
+    "v_and_or_b32 %[vt0], s53, %[ghi], %[vt0]\n\t" \
+    "v_bcnt_u32_b32 %[vt0], %[vt0], 0\n\t" \
+    "v_and_b32 %[vt0], 1, %[vt0]\n\t" \
+    "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t" \
+    "s_and_b64 s[64:65], s[52:53], s[58:59]\n\t" \
+    "s_cbranch_scc0 L_hempty%=\n\t" \
+    "s_ff1_i32_b64 s56, s[64:65]\n\t" \
+    "s_bitset0_b64 s[58:59], s56\n\t" \
+    "v_readlane_b32 s66, %[jlo], s56\n\t" \
+    "v_readlane_b32 s67, %[jhi], s56\n\t" \
+    "v_readlane_b32 s46, %[glo], s56\n\t" \
+    "v_readlane_b32 s47, %[ghi], s56\n\t" \
+    "s_and_b32 s56, s67, 0xe0000000\n\t" \
+    "s_and_b32 s67, s67, 0x1fffffff\n\t" \
+    "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t" \
+    "s_nop 1\n\t" \
+    "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t" \
+    "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t" \
+    "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t" \
+    "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t" \
+    "v_bitop3_b32 %[glo], %[glo], s46, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[ghi], %[ghi], s47, %[vt0] bitop3:0x78\n\t" \
+    "s_bitcmp1_b32 s56, 30\n\t" \
+    "s_cbranch_scc0 L_hnojii%=\n\t" \
+    "v_xor_b32 %[vt3], %[vt3], %[vt0]\n" \
+    "L_hnojii%=:\n\t" \
+    "s_add_u32 s60, s60, 0x10\n\t" \
+    "s_bitcmp1_b32 s51, 0\n\t" \
+    "s_cbranch_scc0 L_hnoc%=\n\t" \
+    "s_or_b32 s65, s65, 0x80000000\n\t" \
+    "s_add_u32 s60, s60, s56\n\t" \
+    "s_xor_b64 s[48:49], s[48:49], s[46:47]\n" \
+    "L_hnoc%=:\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t" \
+    "s_branch L_hr%=\n" \
+    "L_hempty%=:\n\t" \
+    "s_bitcmp1_b32 s51, 0\n\t" \
+    "s_cbranch_scc1 L_zero%=\n\t" \
+    "s_branch L_hr%=\n" \
+    /* couplings */ \
+    "L_hc%=:\n\t" \
+    "s_add_u32 s46, %[jb], m0\n\t" \
+    "s_andn2_b32 s62, %[pmask], s46\n" \
+    "L_hcl%=:\n\t" \
+    "s_cmp_eq_u32 s62, 0\n\t" \
+    "s_cbranch_scc1 L_elim%=\n\t" \
+    "s_ff1_i32_b32 s51, s62\n\t" \
+    "s_bitset0_b32 s62, s51\n\t" \
+    "s_sub_u32 s50, %[sm1], s51\n\t" \
+    "s_lshl_b32 s50, s50, 1\n\t" \
+    "s_lshl_b64 s[52:53], 1, s50\n\t" \
+    "s_lshl_b64 s[46:47], 2, s50\n\t" \
+    "v_and_b32 %[vt0], s52, %[glo]\n\t" \
+    "v_and_or_b32 %[vt0], s53, %[ghi], %[vt0]\n\t" \
+    "v_and_b32 %[vt1], s46, %[glo]\n\t" \
+    "v_and_or_b32 %[vt1], s47, %[ghi], %[vt1]\n\t" \
+    "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t" \
+    "v_cmp_ne_u32_e64 s[46:47], 0, %[vt1]\n\t" \
+    "s_lshr_b64 s[66:67], s[48:49], s50\n\t" \
+    "s_and_b64 s[52:53], s[52:53], s[58:59]\n\t" \
+    "s_and_b64 s[46:47], s[46:47], s[58:59]\n\t" \
+    /* Q += 4 h_u h_v */ \
+    "s_and_b32 s67, s66, 3\n\t" \
+    "s_cmp_eq_u32 s67, 3\n\t" \
+    "s_cselect_b32 s67, 0x80000000, 0\n\t" \
+    "s_add_u32 s60, s60, s67\n\t" \
+    /* lin = (cv & -h_u) ^ (cu & -h_v) ^ (cu & cv) */ \
+    "s_bitcmp1_b32 s66, 0\n\t" \
+    "s_cselect_b64 s[64:65], s[46:47], 0\n\t" \
+    "s_bitcmp1_b32 s66, 1\n\t" \
+    "s_cselect_b64 s[66:67], s[52:53], 0\n\t" \
+    "s_xor_b64 s[64:65], s[64:65], s[66:67]\n\t" \
+    "s_and_b64 s[66:67], s[52:53], s[46:47]\n\t" \
+    "s_xor_b64 s[64:65], s[64:65], s[66:67]\n\t" \
+    "v_cndmask_b32_e64 %[vt0], 0, -1, s[52:53]\n\t" \
+    "v_cndmask_b32_e64 %[vt1], 0, -1, s[46:47]\n\t" \
+    "v_cndmask_b32_e64 %[vt2], 0, -1, s[64:65]\n\t" \
+    "v_and_b32 %[vt2], 0x80000000, %[vt2]\n\t" \
+    "v_xor_b32 %[jhi], %[jhi], %[vt2]\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s46, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s47, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s52, %[vt1] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s53, %[vt1] bitop3:0x78\n\t" \
+    "s_branch L_hcl%=\n" \
+    "L_elim%=:\n" \
+    LOOP \
+    "L_end%=:\n\t" \
+    "s_cmp_eq_u32 s56, 60\n\t" \
+    "s_cselect_b32 s60, s60, 0\n" \
+    "L_emit%=:\n\t" \
+    "v_writelane_b32 %[parked], s60, m0\n\t" \
+    "s_add_u32 m0, m0, 1\n\t" \
+    "s_cmp_lt_u32 m0, %[nb]\n\t" \
+    "s_cbranch_scc1 L_ip%=\n\t" \
+    "s_branch L_done%=\n" \
+    "L_zero%=:\n\t" \
+    "s_mov_b32 s60, 0\n\t" \
+    "s_branch L_emit%=\n" \
+    "L_done%=:\n\t" \
+    "s_mov_b32 m0, s61\n\t" \
+    "s_nop 1\n\t"
+
+template <bool ALL>
 DEV int phiH_batch(unsigned nb, unsigned jb, unsigned jlo0, unsigned jhi0, u64 g0, u64 actR, u64 h0,
                    unsigned wbase, unsigned smask, unsigned sm1, unsigned pmask) {
     int parked;
     unsigned jlo, jhi, vt0, vt1, vt2, vt3, glo, ghi;
-    asm volatile(
-        "v_mov_b32 %[parked], 0\n\t"
-        "v_mov_b32 v40, 0xc0000000\n\t"
-        "s_mov_b32 s61, m0\n\t"
-        "s_mov_b32 m0, 0\n"
-        "L_ip%=:\n\t"
-        "s_add_u32 s46, %[jb], m0\n\t"
-        "s_and_b32 s46, s46, %[smask]\n\t"
-        "s_bcnt1_i32_b32 s47, s46\n\t"
-        "s_lshl_b32 s47, s47, 4\n\t"
-        "s_sub_u32 s60, %[wbase], s47\n\t"
-        "v_mov_b32 %[jlo], %[jlo0]\n\t"
-        "v_mov_b32 %[jhi], %[jhi0]\n\t"
-        "v_mov_b32 %[glo], %[g0lo]\n\t"
-        "v_mov_b32 %[ghi], %[g0hi]\n\t"
-        "s_mov_b64 s[48:49], %[h0]\n\t"
-        "s_mov_b64 s[58:59], %[actR]\n\t"
-        "s_mov_b32 s62, s46\n"
-        // constraints
-        "L_hr%=:\n\t"
-        "s_cmp_eq_u32 s62, 0\n\t"
-        "s_cbranch_scc1 L_hc%=\n\t"
-        "s_ff1_i32_b32 s51, s62\n\t"
-        "s_bitset0_b32 s62, s51\n\t"
-        "s_sub_u32 s50, %[sm1], s51\n\t"
-        "s_lshl_b32 s50, s50, 1\n\t"
-        "s_lshl_b64 s[52:53], 3, s50\n\t"
-        "s_and_b64 s[66:67], s[48:49], s[52:53]\n\t"
-        "s_bcnt1_i32_b64 s51, s[66:67]\n\t"
-        "v_and_b32 %[vt0], s52, %[glo]\n\t"
-        "v_and_or_b32 %[vt0], s53, %[ghi], %[vt0]\n\t"
-        "v_bcnt_u32_b32 %[vt0], %[vt0], 0\n\t"
-        "v_and_b32 %[vt0], 1, %[vt0]\n\t"
-        "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t"
-        "s_and_b64 s[64:65], s[52:53], s[58:59]\n\t"
-        "s_cbranch_scc0 L_hempty%=\n\t"
-        "s_ff1_i32_b64 s56, s[64:65]\n\t"
-        "s_bitset0_b64 s[58:59], s56\n\t"
-        "v_readlane_b32 s66, %[jlo], s56\n\t"
-        "v_readlane_b32 s67, %[jhi], s56\n\t"
-        "v_readlane_b32 s46, %[glo], s56\n\t"
-        "v_readlane_b32 s47, %[ghi], s56\n\t"
-        "s_and_b32 s56, s67, 0xe0000000\n\t"
-        "s_and_b32 s67, s67, 0x1fffffff\n\t"
-        "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t"
-        "s_nop 1\n\t"
-        "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"
-        "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"
-        "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t"
-        "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t"
-        "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t"
-        "v_bitop3_b32 %[glo], %[glo], s46, %[vt0] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[ghi], %[ghi], s47, %[vt0] bitop3:0x78\n\t"
-        "s_bitcmp1_b32 s56, 30\n\t"
-        "s_cbranch_scc0 L_hnojii%=\n\t"
-        "v_xor_b32 %[vt3], %[vt3], %[vt0]\n"
-        "L_hnojii%=:\n\t"
-        "s_add_u32 s60, s60, 0x10\n\t"
-        "s_bitcmp1_b32 s51, 0\n\t"
-        "s_cbranch_scc0 L_hnoc%=\n\t"
-        "s_or_b32 s65, s65, 0x80000000\n\t"
-        "s_add_u32 s60, s60, s56\n\t"
-        "s_xor_b64 s[48:49], s[48:49], s[46:47]\n"
-        "L_hnoc%=:\n\t"
-        "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t"
-        "s_branch L_hr%=\n"
-        "L_hempty%=:\n\t"
-        "s_bitcmp1_b32 s51, 0\n\t"
-        "s_cbranch_scc1 L_zero%=\n\t"
-        "s_branch L_hr%=\n"
-        // couplings
-        "L_hc%=:\n\t"
-        "s_add_u32 s46, %[jb], m0\n\t"
-        "s_andn2_b32 s62, %[pmask], s46\n"
-        "L_hcl%=:\n\t"
-        "s_cmp_eq_u32 s62, 0\n\t"
-        "s_cbranch_scc1 L_elim%=\n\t"
-        "s_ff1_i32_b32 s51, s62\n\t"
-        "s_bitset0_b32 s62, s51\n\t"
-        "s_sub_u32 s50, %[sm1], s51\n\t"
-        "s_lshl_b32 s50, s50, 1\n\t"
-        "s_lshl_b64 s[52:53], 1, s50\n\t"
-        "s_lshl_b64 s[46:47], 2, s50\n\t"
-        "v_and_b32 %[vt0], s52, %[glo]\n\t"
-        "v_and_or_b32 %[vt0], s53, %[ghi], %[vt0]\n\t"
-        "v_and_b32 %[vt1], s46, %[glo]\n\t"
-        "v_and_or_b32 %[vt1], s47, %[ghi], %[vt1]\n\t"
-        "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t"
-        "v_cmp_ne_u32_e64 s[46:47], 0, %[vt1]\n\t"
-        "s_lshr_b64 s[66:67], s[48:49], s50\n\t"
-        "s_and_b64 s[52:53], s[52:53], s[58:59]\n\t"
-        "s_and_b64 s[46:47], s[46:47], s[58:59]\n\t"
-        // Q += 4 h_u h_v
-        "s_and_b32 s67, s66, 3\n\t"
-        "s_cmp_eq_u32 s67, 3\n\t"
-        "s_cselect_b32 s67, 0x80000000, 0\n\t"
-        "s_add_u32 s60, s60, s67\n\t"
-        // lin = (cv & -h_u) ^ (cu & -h_v) ^ (cu & cv)
-        "s_bitcmp1_b32 s66, 0\n\t"
-        "s_cselect_b64 s[64:65], s[46:47], 0\n\t"
-        "s_bitcmp1_b32 s66, 1\n\t"
-        "s_cselect_b64 s[66:67], s[52:53], 0\n\t"
-        "s_xor_b64 s[64:65], s[64:65], s[66:67]\n\t"
-        "s_and_b64 s[66:67], s[52:53], s[46:47]\n\t"
-        "s_xor_b64 s[64:65], s[64:65], s[66:67]\n\t"
-        "v_cndmask_b32_e64 %[vt0], 0, -1, s[52:53]\n\t"
-        "v_cndmask_b32_e64 %[vt1], 0, -1, s[46:47]\n\t"
-        "v_cndmask_b32_e64 %[vt2], 0, -1, s[64:65]\n\t"
-        "v_and_b32 %[vt2], 0x80000000, %[vt2]\n\t"
-        "v_xor_b32 %[jhi], %[jhi], %[vt2]\n\t"
-        "v_bitop3_b32 %[jlo], %[jlo], s46, %[vt0] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[jhi], %[jhi], s47, %[vt0] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[jlo], %[jlo], s52, %[vt1] bitop3:0x78\n\t"
-        "v_bitop3_b32 %[jhi], %[jhi], s53, %[vt1] bitop3:0x78\n\t"
-        "s_branch L_hcl%=\n"
-        "L_elim%=:\n"
-        CSE_ELIM_LOOP_ISO
-        "L_end%=:\n\t"
-        "s_cmp_eq_u32 s56, 60\n\t"
-        "s_cselect_b32 s60, s60, 0\n"
-        "L_emit%=:\n\t"
-        "v_writelane_b32 %[parked], s60, m0\n\t"
-        "s_add_u32 m0, m0, 1\n\t"
-        "s_cmp_lt_u32 m0, %[nb]\n\t"
-        "s_cbranch_scc1 L_ip%=\n\t"
-        "s_branch L_done%=\n"
-        "L_zero%=:\n\t"
-        "s_mov_b32 s60, 0\n\t"
-        "s_branch L_emit%=\n"
-        "L_done%=:\n\t"
-        "s_mov_b32 m0, s61\n\t"
-        "s_nop 1\n\t"
-        : [parked] "=&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
-          [vt2] "=&v"(vt2), [vt3] "=&v"(vt3), [glo] "=&v"(glo), [ghi] "=&v"(ghi)
-        : [nb] "s"(nb), [jb] "s"(jb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [g0lo] "v"((unsigned)g0),
-          [g0hi] "v"((unsigned)(g0 >> 32)), [actR] "s"(actR), [h0] "s"(h0), [wbase] "s"(wbase), [smask] "s"(smask),
-          [sm1] "s"(sm1), [pmask] "s"(pmask)
-        : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
-          "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+    if constexpr (ALL) {
+        asm volatile(
+            PHIH_BODY(CSE_ELIM_LOOP_ISO)
+            : [parked] "=&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
+              [vt2] "=&v"(vt2), [vt3] "=&v"(vt3), [glo] "=&v"(glo), [ghi] "=&v"(ghi)
+            : [nb] "s"(nb), [jb] "s"(jb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [g0lo] "v"((unsigned)g0),
+              [g0hi] "v"((unsigned)(g0 >> 32)), [actR] "s"(actR), [h0] "s"(h0), [wbase] "s"(wbase), [smask] "s"(smask),
+              [sm1] "s"(sm1), [pmask] "s"(pmask)
+            : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
+              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+    } else {
+        asm volatile(
+            PHIH_BODY(CSE_ELIM_LOOP)
+            : [parked] "=&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
+              [vt2] "=&v"(vt2), [vt3] "=&v"(vt3), [glo] "=&v"(glo), [ghi] "=&v"(ghi)
+            : [nb] "s"(nb), [jb] "s"(jb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [g0lo] "v"((unsigned)g0),
+              [g0hi] "v"((unsigned)(g0 >> 32)), [actR] "s"(actR), [h0] "s"(h0), [wbase] "s"(wbase), [smask] "s"(smask),
+              [sm1] "s"(sm1), [pmask] "s"(pmask)
+            : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
+              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+    }
     return parked;
 }
 
@@ -1231,7 +1249,7 @@ DEV void sweep_body(const SweepArgs &A) {
             const int diag = (mode == 1 && r >= jb && r < jb + nb) ? (int)(r - jb) : -1;
             bad |= fold_terms(parked, nb, mode, diag, rv.sexp, acc, hist);
         };
-        if (D.exact && PK && ALL) {
+        if (D.exact && PK) {
             // the whole batch in one asm block
             const unsigned size = (unsigned)((n + 1) / 2);
             const unsigned smask = (1u << size) - 1u;
@@ -1241,7 +1259,7 @@ DEV void sweep_body(const SweepArgs &A) {
             const unsigned jlo0 = (unsigned)rv.J, jhi0 = (unsigned)(rv.J >> 32);
             for (long long jb = j0; jb < j1; jb += 64) {
                 const int nb = (int)(j1 - jb < 64 ? j1 - jb : 64);
-                const int parked = phiL_unpack(phiH_batch(uni32((unsigned)nb), uni32((unsigned)jb), jlo0, jhi0, rv.G,
+                const int parked = phiL_unpack(phiH_batch<ALL>(uni32((unsigned)nb), uni32((unsigned)jb), jlo0, jhi0, rv.G,
                                                           actR, rv.h, uni32(wbase), uni32(smask), uni32(size - 1u),
                                                           uni32(pmask)));
                 flush(parked, nb, jb);
@@ -1386,13 +1404,12 @@ extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_b
 
 // the same sweep with D packed into the J rows (every instance has n <= 60;
 // row 60 is the elimination loop's end row).
-// Eight waves per SIMD (63 VGPRs, 78 SGPRs): the elimination loop is bound by
-// instruction issue, and the extra wave hides more of its VALU -> SALU
-// dependency latency.
+// Eight waves per SIMD (63 VGPRs, 78 SGPRs, one 8-byte spill stored at entry
+// and reloaded at exit): the elimination loop is bound by instruction issue,
+// and the extra wave hides more of its VALU -> SALU dependency latency.
 extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed(SweepArgs A) { sweep_body<true, false>(A); }
 
-// The packed sweep of exact mode and of every launch with |H^t> instances:
-// exact-mode records (projected |L> states of dimension about |x_i|) have
+// The packed sweep of exact mode: exact-mode records (projected |L> states of dimension about |x_i|) have
 // about n/2 non-pivot columns, so every |L> term runs in the asm block with
 // its general restriction loop, and every |H^t> term in phiH_batch.  The working G
 // rows take two more VGPRs: seven waves per SIMD, without spills.
