@@ -50,11 +50,12 @@ def exact_mean(t, sexp, zsum):
     return float(tot / len(sexp))
 
 
-def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0):
+def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0, lib="libref.so", threads=0):
     """The reference's own C (oracle/_ref/libref.so, compiled from the reference
-    sources) on a bounded sample of the same workload: the first nphi
-    decomposition states for `threads` projected thetas, one POSIX thread per
-    theta.  Falls back to the oracle (a port) when libref.so is absent."""
+    sources at -O2; libref_O0.so at -O0) on a bounded sample of the same
+    workload: the first nphi decomposition states for `threads` projected
+    thetas (default: the host's cores, at most 16), one POSIX thread per theta.
+    Falls back to the oracle (a port) when the library is absent."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes
 
@@ -62,7 +63,7 @@ def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0):
     ph, xs, zs = P_tuple
     Po = orc.make_projector(ph, xs, zs, t)
     thr = orc.thresholds(t)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
     thetas = []
     st = 0
     while len(thetas) < threads:
@@ -71,8 +72,8 @@ def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0):
         if orc.project(th, Po) >= 0:
             thetas.append(th)
     Larr = (orc.U64 * k)(*L)
-    if orc.ref_available():
-        R = orc.ref()
+    if orc.ref_available(lib):
+        R = orc.ref(lib)
         arr1 = (orc.OrcState * 1)(thetas[0])
         z = (orc.I64 * 4)()
         t0 = time.perf_counter()
@@ -86,7 +87,8 @@ def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0):
         dt = time.perf_counter() - t0
         return {"value": n / dt, "unit": "inner products/s", "cores": threads, "kind": "reference",
                 "sample": f"{threads} projected thetas x first {nphi} of {1 << k} |L> states at t={t} "
-                          f"({n} inner products, reference stabilizer.c/stateprep.c built -O2, "
+                          f"({n} inner products, reference stabilizer.c/stateprep.c built "
+                          f"{'-O0' if lib == 'libref_O0.so' else '-O2'}, "
                           f"{threads} threads, {dt:.1f} s)"}
     # oracle port, single thread
     phis = [orc.prepL(i, t, L) for i in range(64)]
@@ -271,6 +273,11 @@ def main():
                          "h30: the exact projector with the |H^t> decomposition at t=30")
     ap.add_argument("--shots", type=int, default=1024, help="c5: number of shots")
     ap.add_argument("--instances", type=int, default=4096, help="c3: number of instances")
+    ap.add_argument("--cpu-baseline-only", action="store_true",
+                    help="c4: time only the CPU baseline (no GPU) and print it as one JSON line")
+    ap.add_argument("--ref-opt", choices=["O2", "O0"], default="O2",
+                    help="optimisation level of the reference C build the CPU baseline times")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: cores, at most 16)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -292,6 +299,11 @@ def main():
     t, r, k = 48, 24, 12
     M = args.samples
     P, P_tuple, L = sampled_config(t, r, k, seed=1004)
+    reflib = "libref_O0.so" if args.ref_opt == "O0" else "libref.so"
+    if args.cpu_baseline_only:
+        print(json.dumps({"cpu_baseline": cpu_baseline(P_tuple, L, t, k, 1004, lib=reflib,
+                                                       threads=args.cpu_threads)}), flush=True)
+        return
     chi = 1 << k
     with Engine(0) as E:
         cus, devname = E.device_info()
@@ -370,7 +382,7 @@ def main():
         "hbm_GBps": kernel_ips * SWEEP_BYTES_PER_IP / 1e9,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(P_tuple, L, t, k, 1004)
+        out["cpu_baseline"] = cpu_baseline(P_tuple, L, t, k, 1004, lib=reflib, threads=args.cpu_threads)
     print(json.dumps(out), flush=True)
 
 

@@ -60,7 +60,7 @@ class IpRes(ctypes.Structure):
 
 
 _lib = None
-_ref = None
+_ref = {}
 
 
 def build_oracle():
@@ -105,14 +105,15 @@ def lib():
     return _lib
 
 
-def ref_available():
-    return os.path.exists(os.path.join(ORACLE_DIR, "_ref", "libref.so"))
+def ref_available(name="libref.so"):
+    return os.path.exists(os.path.join(ORACLE_DIR, "_ref", name))
 
 
-def ref():
-    global _ref
-    if _ref is None:
-        R = ctypes.CDLL(os.path.join(ORACLE_DIR, "_ref", "libref.so"))
+def ref(name="libref.so"):
+    """the reference C built from its sources: libref.so (-O2) or
+    libref_O0.so (-O0, the reference Makefile's level)"""
+    if name not in _ref:
+        R = ctypes.CDLL(os.path.join(ORACLE_DIR, "_ref", name))
         P = ctypes.POINTER
         R.ref_innerProduct.argtypes = [P(OrcState), P(OrcState)] + [P(ctypes.c_int)] * 3
         R.ref_exponentialSum.argtypes = [P(OrcState)] + [P(ctypes.c_int)] * 3
@@ -125,8 +126,8 @@ def ref():
         R.ref_sampled_sweep.argtypes = [P(OrcState), ctypes.c_int, ctypes.c_int, P(U64), ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, P(I64)]
         R.ref_sampled_sweep.restype = ctypes.c_long
-        _ref = R
-    return _ref
+        _ref[name] = R
+    return _ref[name]
 
 
 # ---------------------------------------------------------------- helpers
