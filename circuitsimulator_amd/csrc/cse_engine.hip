@@ -324,22 +324,15 @@ int check_decomp(cse_ctx *ctx, int t, int k, int exact, const uint64_t *L) {
     return CSE_OK;
 }
 
-}  // namespace
+// the message of the calling thread's last failed cse_create (no context
+// exists to hold it); cse_error(NULL) returns it
+thread_local char g_create_err[256];
 
-// ====================================================================== ABI
-extern "C" {
+// a state-level call's count (pairs, states) is passed to its kernel as an
+// int, and its grid is count / 4 blocks
+constexpr size_t kMaxStates = size_t(1) << 30;
 
-int cse_create(int device, cse_ctx **out) {
-    if (!out) return CSE_ERR_ARG;
-    *out = nullptr;
-    int count = 0;
-    hipError_t e = hipGetDeviceCount(&count);
-    if (e != hipSuccess || count == 0) {
-        (void)hipGetLastError();
-        return CSE_ERR_HIP;
-    }
-    if (device < 0 || device >= count) return CSE_ERR_ARG;
-    cse_ctx *ctx = new cse_ctx();
+int init_ctx(cse_ctx *ctx, int device) {
     ctx->device = device;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -350,8 +343,39 @@ int cse_create(int device, cse_ctx **out) {
     HIPCHK(hipGetDeviceProperties(&prop, device));
     ctx->cus = prop.multiProcessorCount;
     snprintf(ctx->name, sizeof(ctx->name), "%s (%s)", prop.name, prop.gcnArchName);
-    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
-        int rc = fail(ctx, CSE_ERR_HIP, "device %d is %s; this engine is built for gfx950", device, prop.gcnArchName);
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(ctx, CSE_ERR_HIP, "device %d is %s; this engine is built for gfx950", device, prop.gcnArchName);
+    return CSE_OK;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+int cse_create(int device, cse_ctx **out) {
+    g_create_err[0] = 0;
+    if (!out) {
+        snprintf(g_create_err, sizeof(g_create_err), "cse_create: NULL output pointer");
+        return CSE_ERR_ARG;
+    }
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        snprintf(g_create_err, sizeof(g_create_err), "cse_create: no HIP device (%s)",
+                 e != hipSuccess ? hipGetErrorString(e) : "device count 0");
+        return CSE_ERR_HIP;
+    }
+    if (device < 0 || device >= count) {
+        snprintf(g_create_err, sizeof(g_create_err), "cse_create: device %d outside [0, %d)", device, count);
+        return CSE_ERR_ARG;
+    }
+    cse_ctx *ctx = new cse_ctx();
+    const int rc = init_ctx(ctx, device);
+    if (rc != CSE_OK) {
+        snprintf(g_create_err, sizeof(g_create_err), "cse_create: %s", ctx->err);
         cse_destroy(ctx);
         return rc;
     }
@@ -377,7 +401,10 @@ void cse_destroy(cse_ctx *ctx) {
     delete ctx;
 }
 
-const char *cse_error(const cse_ctx *ctx) { return ctx ? ctx->err : "no context"; }
+const char *cse_error(const cse_ctx *ctx) {
+    if (ctx) return ctx->err;
+    return g_create_err[0] ? g_create_err : "no context";
+}
 
 int cse_device_info(cse_ctx *ctx, int *cus, char *name, int name_len) {
     if (!ctx) return CSE_ERR_ARG;
@@ -395,6 +422,7 @@ int cse_inner_products(cse_ctx *ctx, int n, size_t npairs, const uint64_t *s1, c
         return CSE_OK;
     }
     if (!s1 || !s2 || !out) return fail(ctx, CSE_ERR_ARG, "NULL buffer");
+    if (npairs > kMaxStates) return fail(ctx, CSE_ERR_ARG, "%zu pairs above 2^30 per call", npairs);
     const size_t words = (size_t)CSE_STATE_WORDS(n);
     for (size_t i = 0; i < npairs; i++) {
         const uint64_t *a = s1 + words * i, *b = s2 + words * i;
@@ -436,6 +464,7 @@ int cse_prep_states(cse_ctx *ctx, int exact, int t, const uint64_t *L, int k, co
     int rc = check_decomp(ctx, t, k, exact, L);
     if (rc) return rc;
     if (!idx || !out) return fail(ctx, CSE_ERR_ARG, "NULL buffer");
+    if (count > kMaxStates) return fail(ctx, CSE_ERR_ARG, "%zu states above 2^30 per call", count);
     const uint64_t chi = exact ? (1ull << ((t + 1) / 2)) : (1ull << k);
     for (size_t c = 0; c < count; c++)
         if (idx[c] >= chi) return fail(ctx, CSE_ERR_ARG, "index %llu >= chi", (unsigned long long)idx[c]);
@@ -459,6 +488,7 @@ int cse_random_states(cse_ctx *ctx, int n, uint64_t seed, int64_t first, size_t 
     if (!ctx) return CSE_ERR_ARG;
     if (n < 1 || n > CSE_NMAX) return fail(ctx, CSE_ERR_ARG, "n=%d outside [1, %d]", n, CSE_NMAX);
     if (count == 0) return CSE_OK;
+    if (count > kMaxStates) return fail(ctx, CSE_ERR_ARG, "%zu states above 2^30 per call", count);
     if (P) {
         int rc = check_projector(ctx, P);
         if (rc) return rc;

@@ -150,7 +150,7 @@ class Engine:
         h = ctypes.c_void_p()
         rc = self._L.cse_create(device, ctypes.byref(h))
         if rc != 0:
-            raise EngineError(f"cse_create failed ({rc}): no usable gfx950 device {device}")
+            raise EngineError(f"cse_create failed ({rc}): {self._L.cse_error(None).decode()}")
         self.h = h
 
     def close(self):

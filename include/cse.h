@@ -5,7 +5,8 @@
  * thread; every call is synchronous on the context's HIP stream.
  *
  * Return values: CSE_OK (0) or a negative CSE_ERR_* code; cse_error() gives
- * the message of the last failure on the context.
+ * the message of the last failure on the context, cse_error(NULL) that of the
+ * calling thread's last failed cse_create.
  */
 #ifndef CSE_H
 #define CSE_H

@@ -35,8 +35,19 @@ def test_no_gpu_create_fails_cleanly():
     if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU is present")
     from circuitsimulator_amd import Engine, EngineError
-    with pytest.raises(EngineError):
+    with pytest.raises(EngineError, match="cse_create failed .*: cse_create: "):
         Engine(0)
+    # the C ABI keeps the reason for cse_error(NULL); a NULL output pointer
+    # is its own error
+    lib = ctypes.CDLL(LIB)
+    lib.cse_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    lib.cse_error.argtypes = [ctypes.c_void_p]
+    lib.cse_error.restype = ctypes.c_char_p
+    h = ctypes.c_void_p()
+    assert lib.cse_create(0, ctypes.byref(h)) < 0 and not h.value
+    assert lib.cse_error(None).startswith(b"cse_create: ")
+    assert lib.cse_create(0, None) < 0
+    assert lib.cse_error(None) == b"cse_create: NULL output pointer"
 
 
 def test_struct_layouts_match_the_header(tmp_path):

@@ -349,6 +349,30 @@ def test_batch_sampled_rejects_bad_counts(E):
         E.batch_sampled(A)
 
 
+def test_state_calls_reject_counts_above_the_limit(E):
+    # at most 2^30 pairs or states per call, checked on the host before any
+    # buffer is read or anything is launched (the buffers here hold one)
+    import ctypes
+
+    from circuitsimulator_amd.engine import U64, _ptr
+    L = E._L
+    big = (1 << 30) + 1
+    st = np.zeros(3 * 4 + 4, dtype=np.uint64)
+    out = np.zeros(16, dtype=np.int32)
+    assert L.cse_inner_products(E.h, 4, big, _ptr(st, U64), _ptr(st, U64), _ptr(out, ctypes.c_int32), 0,
+                                None) < 0
+    assert b"2^30" in L.cse_error(E.h)
+    idx = np.zeros(1, dtype=np.uint64)
+    assert L.cse_prep_states(E.h, 1, 4, None, 0, _ptr(idx, U64), big, _ptr(st, U64)) < 0
+    assert b"2^30" in L.cse_error(E.h)
+    sexp = np.zeros(1, dtype=np.int32)
+    assert L.cse_random_states(E.h, 4, 1, 0, big, None, None, _ptr(sexp, ctypes.c_int32)) < 0
+    assert b"2^30" in L.cse_error(E.h)
+    # the context still works
+    res = E.prep_states(1, 4, [], [0, 3])
+    assert res.shape == (2, 16)
+
+
 @pytest.mark.parametrize("bins", [1, 3, 4])
 def test_projector_norm_bins_in_one_call(E, bins):
     # cse_projector_norm sends every bin as one instance of one batched call;
