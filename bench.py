@@ -103,24 +103,54 @@ def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0, lib="libref.so", threads
             "sample": f"{n} inner products theta' x prepL at t={t} by the oracle, 1 thread, {dt:.1f} s"}
 
 
+def chunk_arg(args, default):
+    """--chunk: an item size in j, 0 for the engine's rule, or "auto" for the
+    autotuner's choice; `default` where it is not given"""
+    v = args.chunk if args.chunk is not None else default
+    return v if v == "auto" else int(v)
+
+
+def tuned_exact_chunk(E, instances, arg):
+    """(item size, tuning record): the autotuner runs before the warmup, outside
+    the timed region, on a proxy of the same instances"""
+    if arg != "auto":
+        return arg, {"size": arg}
+    from circuitsimulator_amd.autotune import tune_exact_chunk
+    tm = {}
+    t0 = time.perf_counter()
+    c = tune_exact_chunk(E, instances, timings=tm)
+    return c, {"size": c, "tune_s": time.perf_counter() - t0,
+               "proxy_sweep_ms": {str(key): round(v, 4) for key, v in tm.items()}}
+
+
 def exact_pairs_config(args):
     """C2, reading B (the reference-native loop, innerprod.c:148-261): exact
     mode at t=24, r=12, k=11, chi=2048, so 2,098,176 pair terms, each one
-    inner product <phi_j|Pi phi_i>.  value = pair terms/s of the whole call."""
+    inner product <phi_j|Pi phi_i>, as one cse_batch_exact instance (what
+    cse_exact_projector runs) with the autotuner's item size.  value = pair
+    terms/s of the whole call."""
     from circuitsimulator_amd import Engine
+    from circuitsimulator_amd.engine import Instance
     from circuitsimulator_amd.workloads import sampled_config
     t, r, k = 24, 12, 11
     P, P_tuple, L = sampled_config(t, r, k, seed=1002)
     chi = 1 << k
     nterms = chi * (chi + 1) // 2
+    inst = Instance()
+    inst.P = P
+    inst.k = k
+    for i, v in enumerate(L):
+        inst.L[i] = v
     with Engine(0) as E:
+        chunk, tuning = tuned_exact_chunk(E, [inst], chunk_arg(args, "auto"))
         for _ in range(args.warmup):
-            E.exact_projector(P, L, k, 0)
+            E.batch_exact([inst], chunk=chunk)
         t0 = time.perf_counter()
         kernel_ms = 0.0
         for _ in range(args.steps):
-            res, _ = E.exact_projector(P, L, k, 0)
-            kernel_ms += res.kernel_ms
+            rl, _, kms = E.batch_exact([inst], chunk=chunk)
+            res = rl[0]
+            kernel_ms += kms
         dt = time.perf_counter() - t0
     return {
         "metric": "exact projector pair terms/s (one inner product each), t=24 r=12 k=11",
@@ -128,7 +158,8 @@ def exact_pairs_config(args):
         "warmup": args.warmup, "ms_per_step": dt * 1000.0 / args.steps, "higher_is_better": True,
         "dtype": "int64", "data": "synthetic (seeded commuting Pauli group and L)",
         "config": {"workload": f"C2 reading B: exact projector t={t} r={r} k={k}, {nterms} pair terms",
-                   "t": t, "r": r, "k": k, "chi": chi, "seed": 1002},
+                   "t": t, "r": r, "k": k, "chi": chi, "seed": 1002, "chunk": chunk},
+        "tuning": tuning,
         "kernel_ip_per_s": nterms * args.steps / (kernel_ms / 1000.0),
         "result": {"value_int64_exact": res.value, "re": res.re, "im": res.im,
                    "kahan_fp32": [res.kahan_re, res.kahan_im],
@@ -178,12 +209,13 @@ def batch_config(args):
     inst, specs = uneven_instances(count, seed=1003)
     nterms = sum((1 << k) * ((1 << k) + 1) // 2 for (_, _, k, *_rest) in specs)
     with Engine(0) as E:
+        chunk, tuning = tuned_exact_chunk(E, inst, chunk_arg(args, "auto"))
         for _ in range(args.warmup):
-            E.batch_exact(inst)
+            E.batch_exact(inst, chunk=chunk)
         t0 = time.perf_counter()
         kernel_ms = 0.0
         for _ in range(args.steps):
-            res, _, ms = E.batch_exact(inst)
+            res, _, ms = E.batch_exact(inst, chunk=chunk)
             kernel_ms += ms
         dt = time.perf_counter() - t0
     ims = [abs(r.im) for r in res]
@@ -193,7 +225,8 @@ def batch_config(args):
         "warmup": args.warmup, "ms_per_step": dt * 1000.0 / args.steps, "higher_is_better": True,
         "dtype": "int64", "data": "synthetic (seeded per-instance commuting Pauli groups and L)",
         "config": {"workload": f"C3: {count} instances t~U{{8..48}} r~U{{1..t/2}} k~U{{2..14}}, "
-                               f"{nterms} pair terms", "instances": count, "seed": 1003},
+                               f"{nterms} pair terms", "instances": count, "seed": 1003, "chunk": chunk},
+        "tuning": tuning,
         "kernel_ip_per_s": nterms * args.steps / (kernel_ms / 1000.0),
         "checks": {"max_abs_im": max(ims), "min_value": min(r.re for r in res),
                    "terms_reported": int(sum(r.terms for r in res)), "terms_expected": nterms},
@@ -265,7 +298,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--samples", type=int, default=65536, help="random thetas per step (M)")
-    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--chunk", default=None,
+                    help="item size in j, 0 for the engine's rule, or auto for the autotuner's choice "
+                         "(default: auto for c2 and c3, 0 otherwise)")
     ap.add_argument("--plain-pairs", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5", "h30"], default="c4",
@@ -307,14 +342,30 @@ def main():
     chi = 1 << k
     with Engine(0) as E:
         cus, devname = E.device_info()
+        chunk = chunk_arg(args, 0)
+        tuning = {"size": chunk}
+        if chunk == "auto":
+            # the same draw as one step, as a one-instance sampled batch
+            from circuitsimulator_amd.autotune import tune_sampled_chunk
+            from circuitsimulator_amd.engine import sampled_instance_array
+            A = sampled_instance_array(1)
+            A[0]["P"] = np.frombuffer(bytes(P), dtype=A.dtype["P"])[0]
+            A[0]["k"] = k
+            A[0]["L"][:k] = L
+            A[0]["seed"], A[0]["count"] = 1004, M
+            tm = {}
+            t1 = time.perf_counter()
+            chunk = tune_sampled_chunk(E, A, timings=tm)
+            tuning = {"size": chunk, "tune_s": time.perf_counter() - t1,
+                      "proxy_sweep_ms": {str(key): round(v, 4) for key, v in tm.items()}}
         for w in range(args.warmup):
-            E.sampled_projector(P, L, k, 0, 1004 + 100 + w, 0, M, chunk=args.chunk, want_samples=False)
+            E.sampled_projector(P, L, k, 0, 1004 + 100 + w, 0, M, chunk=chunk, want_samples=False)
         ips = 0
         kernel_ms = prep_ms = 0.0
         t0 = time.perf_counter()
         for s in range(args.steps):
             last = s == args.steps - 1
-            res, ex = E.sampled_projector(P, L, k, 0, 1004 + s, 0, M, chunk=args.chunk, want_samples=last)
+            res, ex = E.sampled_projector(P, L, k, 0, 1004 + s, 0, M, chunk=chunk, want_samples=last)
             ips += res.ips
             kernel_ms += res.kernel_ms
             prep_ms += res.prep_ms
@@ -366,6 +417,8 @@ def main():
         "estimator": {"int64_exact": res.mean, "kahan_fp32": res.mean_kahan, "fp64_oracle": oracle,
                       "abs_err_int64": err_int64, "abs_err_kahan_fp32": err_kahan},
     }
+    if "tune_s" in tuning:
+        out["tuning"] = tuning
     # integer-VALU roofline of the sweep kernel (int32 lane-operations/s)
     achieved = kernel_ips * VALU_LANE_OPS_PER_IP / 1e12
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_TOPS, "unit": "Tops/s",

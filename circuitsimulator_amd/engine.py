@@ -244,7 +244,13 @@ class Engine:
         return hist
 
     def batch_exact(self, instances, chunk=0, want_terms=False, nterms=0):
-        """instances: a list of Instance or an instance_array."""
+        """instances: a list of Instance or an instance_array.  chunk: the item
+        size in j (0: the engine's default; "auto": the autotuner's choice)."""
+        if isinstance(chunk, str):
+            if chunk != "auto":
+                raise ValueError(f"chunk must be an int or 'auto', not {chunk!r}")
+            from .autotune import tune_exact_chunk
+            chunk = tune_exact_chunk(self, instances)
         if isinstance(instances, np.ndarray):
             if instances.dtype != INSTANCE_DTYPE or not instances.flags["C_CONTIGUOUS"]:
                 raise ValueError("instances must be a contiguous instance_array")
@@ -262,9 +268,16 @@ class Engine:
     def batch_sampled(self, instances, chunk=0, want_samples=False):
         """instances: a contiguous sampled_instance_array -> (means, results,
         kernel ms, samples); means[i] is cse_sampled_projector's mean for
-        instance i; samples[i] (want_samples) its per-sample zsum and sexp."""
+        instance i; samples[i] (want_samples) its per-sample zsum and sexp.
+        chunk: the item size in j (0: the engine's default; "auto": the
+        autotuner's choice)."""
         if instances.dtype != SAMPLED_INSTANCE_DTYPE or not instances.flags["C_CONTIGUOUS"]:
             raise ValueError("instances must be a contiguous sampled_instance_array")
+        if isinstance(chunk, str):
+            if chunk != "auto":
+                raise ValueError(f"chunk must be an int or 'auto', not {chunk!r}")
+            from .autotune import tune_sampled_chunk
+            chunk = tune_sampled_chunk(self, instances)
         n = len(instances)
         res = (SampledResult * n)()
         samples = []
