@@ -377,22 +377,33 @@ DEV void store_rref(const AState &s, int n, int sexp, uint64_t *rec) {
 //         instance's Philox stream (seed)
 // mode 1: phi_i = prepH(i) (exact) or prepL(i) (|L>), i = local index
 // then Pi (innerprod.c:95-116 / :218-241) and the elimination-ready record.
+// The instance that holds global index key: the largest i < n with
+// off[i] <= key, for nondecreasing off with off[0] <= key.  The whole wave
+// searches: each round lane x reads off[lo + x*step] and the ballot's highest
+// lane narrows [lo, lo + len) to one step, so up to 4096 instances take two
+// dependent loads instead of twelve.  Lane 0 reads off[lo] <= key, true by
+// induction; it enters the ballot unconditionally, so no table can drive lo
+// below 0.  Every lane must be active.
+DEV int find_instance(const long long *__restrict__ off, int n, long long key) {
+    const int x = lane_id();
+    int lo = 0, len = n;
+    while (len > 1) {
+        const int step = (len + 63) >> 6;
+        const bool ok = x * step < len && off[lo + x * step] <= key;
+        const int hl = 63 - __clzll((long long)(ballot(ok) | 1ull));
+        lo += hl * step;
+        len = min(step, len - hl * step);
+    }
+    return lo;
+}
+
 extern "C" __global__ void __launch_bounds__(256)
 k_records(int mode, int ninst, const long long *__restrict__ rec_off, const InstDesc *__restrict__ inst,
           const DevProjector *__restrict__ projs, const uint64_t *__restrict__ thr_table, long long nrec,
           uint64_t *__restrict__ recs, int32_t *__restrict__ sexp_out, int *__restrict__ err) {
     const long long g = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (g >= nrec) return;
-    int in = 0;
-    if (ninst > 1) {
-        int lo = 0, hi = ninst - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (rec_off[mid] <= g) lo = mid;
-            else hi = mid - 1;
-        }
-        in = lo;
-    }
+    const int in = ninst > 1 ? find_instance(rec_off, ninst, g) : 0;
     const InstDesc &D = inst[in];
     const long long local = g - rec_off[in];
     const int n = D.n;
@@ -1069,16 +1080,6 @@ DEV void tri_item(long long l, long long ch, long long nch, long long &r, long l
     const long long off = l - tri_before(ch, nch, g), per = nch - g;
     r = g * ch + off / per;
     c = g + off % per;
-}
-
-DEV int find_instance(const long long *item_off, int ninst, long long item) {
-    int lo = 0, hi = ninst - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (item_off[mid] <= item) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
 }
 
 // FP32 value of 2^{P/2} e^{i pi m/4}

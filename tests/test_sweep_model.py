@@ -19,8 +19,9 @@ supp(x_j) a restriction, columns from working G rows, the batch of empty
 first restrictions, the loop's batch of uncoupled variables), the |H^t>
 block phiH_batch (pair constraints and couplings), the fallback of records
 with more non-pivot columns (restrictions in C++, then the same loop), and
-the exact-mode record built directly in echelon form (rs_project_rref).
-Every term is checked against the oracle's innerProduct (libcirc/
+the exact-mode record built directly in echelon form (rs_project_rref),
+and the wave's 64-ary search for the instance of a record or item
+(find_instance).  Every term is checked against the oracle's innerProduct (libcirc/
 stabilizer/stabilizer.c:589-667), every directly built record against the
 oracle's projection, and the step kinds are counted so that every branch is
 known to be exercised.
@@ -855,3 +856,37 @@ def test_phiH_branches():
                 assert decode(unpack(batch_ip_H(rec, j, t, count))) == ((e, p, m % 8) if e else (0, 0, 0))
     for kind in ("h_restrict", "h_couple", "h_vanish", "A", "B", "C", "iso_batch"):
         assert count[kind] > 0, (kind, dict(count))
+
+
+def wave_find_instance(off, n, key):
+    """find_instance (cse_kernels.hip): lane x of each round reads
+    off[lo + x*step]; the ballot's highest set lane (lane 0 always set)
+    narrows [lo, lo + len) to one step.  Returns (index, rounds)."""
+    lo, length, rounds = 0, n, 0
+    while length > 1:
+        step = (length + 63) >> 6
+        ball = 1
+        for x in range(64):
+            if x * step < length and off[lo + x * step] <= key:
+                ball |= 1 << x
+        hl = ball.bit_length() - 1
+        lo += hl * step
+        length = min(step, length - hl * step)
+        rounds += 1
+    return lo, rounds
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 100, 2048, 4096, 4097, 70000])
+def test_wave_instance_search_matches_bisect(n):
+    import bisect
+    rng = np.random.default_rng(n)
+    off = [0] + [int(v) for v in np.cumsum(rng.integers(1, 50, size=n))]
+    keys = [int(v) for v in rng.integers(0, off[-1], size=200)] + [0, off[-1] - 1] + off[:min(n, 80)]
+    worst = 0
+    for key in keys:
+        got, rounds = wave_find_instance(off, n, key)
+        assert got == bisect.bisect_right(off, key) - 1, (n, key)
+        worst = max(worst, rounds)
+    # two dependent loads up to 4096 instances, three up to 262144
+    assert worst <= (0 if n == 1 else 1 if n <= 64 else 2 if n <= 4096 else 3)
+
