@@ -35,8 +35,13 @@ struct cse_ctx {
     int cus = 0;
     char name[128] = {0};
     char err[512] = {0};
-    DevBuf s1, s2, out, recs, terms, zsum, kahan, hist, ips, queue, itemoff, inst, projs, thr, sexp, L,
-        idx, errflag, recoff, lsh;
+    DevBuf s1, s2, out, recs, terms, zsum, kahan, hist, ips, queue, thr, sexp, L, idx, errflag, lsh;
+    // a batch's plan: instance descriptors, item and record offsets and
+    // projectors in one allocation, laid out as the pinned staging buffer
+    DevBuf plan;
+    InstDesc *d_inst = nullptr;
+    long long *d_itemoff = nullptr, *d_recoff = nullptr;
+    DevProjector *d_projs = nullptr;
     HostBuf hstage;  // results coming back (per-sample sums, histograms)
     HostBuf hup;     // the plan going up
     bool thr_ready = false;
@@ -106,6 +111,24 @@ int zero64(cse_ctx *ctx, void *p, long long words) {
     int blocks = (int)((words + 255) / 256);
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_zero64, dim3(blocks), dim3(256), 0, ctx->stream, (unsigned long long *)p, words);
+    HIPCHK(hipGetLastError());
+    return CSE_OK;
+}
+
+// buffers a batch clears before its launches, in one k_zero_list launch
+void zl_add(ZeroList &zl, void *p, long long words) {
+    if (words <= 0) return;
+    zl.p[zl.count] = (unsigned long long *)p;
+    zl.n[zl.count] = words;
+    zl.count++;
+}
+
+int zero_list(cse_ctx *ctx, const ZeroList &zl) {
+    long long most = 0;
+    for (int b = 0; b < zl.count; b++) most = std::max(most, zl.n[b]);
+    if (most == 0) return CSE_OK;
+    const int blocks = (int)std::min<long long>(4096, (most + 255) / 256);
+    hipLaunchKernelGGL(k_zero_list, dim3(blocks), dim3(256), 0, ctx->stream, zl);
     HIPCHK(hipGetLastError());
     return CSE_OK;
 }
@@ -193,44 +216,50 @@ struct Plan {
     long long nrec = 0, nitems = 0, nterms = 0;
 };
 
-// The plan goes up through one pinned buffer (single DMA transfers); every
-// call synchronizes its stream before returning, so the buffer is free
-// again at the next call.
+// The plan goes up through one pinned buffer in one DMA transfer, into one
+// device allocation of the same layout (256-byte aligned parts); every call
+// synchronizes its stream before returning, so the buffer is free again at
+// the next call.
 int upload_plan(cse_ctx *ctx, const Plan &pl) {
     const int ninst = (int)pl.inst.size();
-    ENSURE(inst, sizeof(InstDesc) * ninst);
-    ENSURE(itemoff, sizeof(long long) * (ninst + 1));
-    ENSURE(recoff, sizeof(long long) * (ninst + 1));
-    ENSURE(projs, sizeof(DevProjector) * ninst);
     const size_t sz[4] = {sizeof(InstDesc) * ninst, sizeof(long long) * (ninst + 1),
                           sizeof(long long) * (ninst + 1), sizeof(DevProjector) * ninst};
     const void *src[4] = {pl.inst.data(), pl.item_off.data(), pl.rec_off.data(), pl.projs.data()};
-    void *dst[4] = {ctx->inst.p, ctx->itemoff.p, ctx->recoff.p, ctx->projs.p};
-    size_t total = 0;
-    for (int b = 0; b < 4; b++) total += (sz[b] + 255) & ~(size_t)255;
+    size_t off[4], total = 0;
+    for (int b = 0; b < 4; b++) {
+        off[b] = total;
+        total += (sz[b] + 255) & ~(size_t)255;
+    }
+    ENSURE(plan, total);
     int rc = ensure_host(ctx, ctx->hup, total);
     if (rc) return rc;
     char *h = (char *)ctx->hup.p;
-    for (int b = 0; b < 4; b++) {
-        memcpy(h, src[b], sz[b]);
-        HIPCHK(hipMemcpyAsync(dst[b], h, sz[b], hipMemcpyHostToDevice, ctx->stream));
-        h += (sz[b] + 255) & ~(size_t)255;
-    }
+    for (int b = 0; b < 4; b++) memcpy(h + off[b], src[b], sz[b]);
+    HIPCHK(hipMemcpyAsync(ctx->plan.p, h, total, hipMemcpyHostToDevice, ctx->stream));
+    char *d = (char *)ctx->plan.p;
+    ctx->d_inst = (InstDesc *)(d + off[0]);
+    ctx->d_itemoff = (long long *)(d + off[1]);
+    ctx->d_recoff = (long long *)(d + off[2]);
+    ctx->d_projs = (DevProjector *)(d + off[3]);
+    return CSE_OK;
+}
+
+// device buffers of the record launch; its error flag joins the batch's clear
+int prep_records(cse_ctx *ctx, const Plan &pl, ZeroList &zl) {
+    ENSURE(recs, (size_t)REC_WORDS * 8 * (size_t)pl.nrec);
+    ENSURE(sexp, 4 * (size_t)pl.nrec);
+    ENSURE(errflag, 8);
+    zl_add(zl, ctx->errflag.p, 1);
     return CSE_OK;
 }
 
 int run_records(cse_ctx *ctx, const Plan &pl, int mode) {
     const int ninst = (int)pl.inst.size();
-    ENSURE(recs, (size_t)REC_WORDS * 8 * (size_t)pl.nrec);
-    ENSURE(sexp, 4 * (size_t)pl.nrec);
-    ENSURE(errflag, 8);
-    int rc0 = zero64(ctx, ctx->errflag.p, 1);
-    if (rc0) return rc0;
     const long long waves = pl.nrec;
     const long long blocks = (waves + 3) / 4;
     if (blocks > 0) {
         hipLaunchKernelGGL(k_records, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, mode, ninst,
-                           dp<long long>(ctx->recoff), dp<InstDesc>(ctx->inst), dp<DevProjector>(ctx->projs),
+                           ctx->d_recoff, ctx->d_inst, ctx->d_projs,
                            dp<uint64_t>(ctx->thr), pl.nrec, dp<uint64_t>(ctx->recs),
                            dp<int32_t>(ctx->sexp), dp<int>(ctx->errflag));
         HIPCHK(hipGetLastError());
@@ -238,7 +267,9 @@ int run_records(cse_ctx *ctx, const Plan &pl, int mode) {
     return CSE_OK;
 }
 
-int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
+// device buffers of the sweep; its sums, counters and queue join the
+// batch's clear
+int prep_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms, ZeroList &zl) {
     const int ninst = (int)pl.inst.size();
     ENSURE(queue, 8);
     ENSURE(ips, 8 * (size_t)ninst);
@@ -247,25 +278,37 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     if (mode == 0) {
         ENSURE(zsum, 32 * (size_t)pl.nrec);
         ENSURE(kahan, 16 * (size_t)pl.nrec);
-        int rc = zero64(ctx, ctx->zsum.p, 4 * pl.nrec);
-        if (!rc) rc = zero64(ctx, ctx->kahan.p, 2 * pl.nrec);
-        if (rc) return rc;
+        zl_add(zl, ctx->zsum.p, 4 * pl.nrec);
+        zl_add(zl, ctx->kahan.p, 2 * pl.nrec);
     } else {
         ENSURE(kahan, 16 * (size_t)ninst);
-        int rc = zero64(ctx, ctx->kahan.p, 2 * (long long)ninst);
-        if (rc) return rc;
+        zl_add(zl, ctx->kahan.p, 2 * (long long)ninst);
     }
-    int rc = zero64(ctx, ctx->queue.p, 1);
-    if (!rc) rc = zero64(ctx, ctx->ips.p, ninst);
-    if (!rc && mode == 1) rc = zero64(ctx, ctx->hist.p, (long long)HIST_STRIDE * ninst);
-    if (rc) return rc;
+    zl_add(zl, ctx->queue.p, 1);
+    zl_add(zl, ctx->ips.p, ninst);
+    if (mode == 1) zl_add(zl, ctx->hist.p, (long long)HIST_STRIDE * ninst);
     if (want_terms) ENSURE(terms, 4 * (size_t)(pl.nterms > 0 ? pl.nterms : 1));
+    return CSE_OK;
+}
+
+// the batch's buffers, cleared in one launch: prep_records, then prep_sweep
+int prep_batch(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
+    ZeroList zl;
+    memset(&zl, 0, sizeof(zl));
+    int rc = prep_records(ctx, pl, zl);
+    if (!rc) rc = prep_sweep(ctx, pl, mode, want_terms, zl);
+    if (!rc) rc = zero_list(ctx, zl);
+    return rc;
+}
+
+int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
+    const int ninst = (int)pl.inst.size();
     SweepArgs A;
     A.mode = mode;
     A.ninst = ninst;
     A.nitems = pl.nitems;
-    A.item_off = dp<long long>(ctx->itemoff);
-    A.inst = dp<InstDesc>(ctx->inst);
+    A.item_off = ctx->d_itemoff;
+    A.inst = ctx->d_inst;
     A.recs = dp<uint64_t>(ctx->recs);
     A.queue = dp<unsigned long long>(ctx->queue);
     A.terms = want_terms ? dp<int32_t>(ctx->terms) : nullptr;
@@ -388,8 +431,8 @@ void cse_destroy(cse_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->s1, &ctx->s2, &ctx->out, &ctx->recs, &ctx->terms, &ctx->zsum, &ctx->kahan,
-                      &ctx->hist, &ctx->ips, &ctx->queue, &ctx->itemoff, &ctx->inst, &ctx->projs, &ctx->thr,
-                      &ctx->sexp, &ctx->L, &ctx->idx, &ctx->errflag, &ctx->recoff, &ctx->lsh};
+                      &ctx->hist, &ctx->ips, &ctx->queue, &ctx->thr, &ctx->sexp, &ctx->L, &ctx->idx,
+                      &ctx->errflag, &ctx->lsh, &ctx->plan};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->hstage.p) (void)hipHostFree(ctx->hstage.p);
@@ -544,6 +587,7 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
     if (want_terms && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
     int rc = ensure_thresholds(ctx);
     if (!rc) rc = upload_plan(ctx, pl);
+    if (!rc) rc = prep_batch(ctx, pl, 0, want_terms);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
     rc = run_records(ctx, pl, 0);
@@ -674,6 +718,7 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
     if (terms_out && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
     int rc = ensure_thresholds(ctx);
     if (!rc) rc = upload_plan(ctx, pl);
+    if (!rc) rc = prep_batch(ctx, pl, 1, terms_out != nullptr);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
     rc = run_records(ctx, pl, 1);

@@ -1454,3 +1454,18 @@ extern "C" __global__ void k_zero64(unsigned long long *p, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0ull;
 }
+
+// The per-call sums, counters and flags of a batch (up to 8 buffers of
+// 64-bit words) cleared in one launch, grid-stride over each buffer.
+struct ZeroList {
+    unsigned long long *p[8];
+    long long n[8];
+    int count;
+};
+
+extern "C" __global__ void k_zero_list(ZeroList L) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int b = 0; b < L.count; b++)
+        for (long long i = i0; i < L.n[b]; i += stride) L.p[b][i] = 0ull;
+}
