@@ -373,6 +373,39 @@ def test_state_calls_reject_counts_above_the_limit(E):
     assert res.shape == (2, 16)
 
 
+def test_reused_buffers_start_clean(E):
+    # a large batch, then a smaller one on the same context: the one clear
+    # launch resets every sum, counter, queue and flag the smaller call
+    # reads, so it matches the same call on a fresh context
+    from circuitsimulator_amd import Engine
+    from circuitsimulator_amd.engine import sampled_instance_array
+    from circuitsimulator_amd.workloads import sampled_config, uneven_instances
+    big, _ = uneven_instances(30, seed=31, kmax=9)
+    small, _ = uneven_instances(5, seed=32, kmax=6)
+
+    def sampled(specs):
+        A = sampled_instance_array(len(specs))
+        for i, (t, r, k, count) in enumerate(specs):
+            P, _, L = sampled_config(t, r, k, seed=500 + i)
+            A[i]["P"] = np.frombuffer(bytes(P), dtype=A.dtype["P"])[0]
+            A[i]["k"] = k
+            A[i]["L"][:k] = L
+            A[i]["seed"], A[i]["count"] = 40 + i, count
+        return A
+    sbig = sampled([(30, 8, 9, 400), (20, 6, 7, 300), (40, 10, 8, 200)])
+    ssmall = sampled([(12, 4, 5, 7), (16, 5, 6, 9)])
+    E.batch_exact(big)
+    after, _, _ = E.batch_exact(small)
+    E.batch_sampled(sbig)
+    safter, sres, _, _ = E.batch_sampled(ssmall)
+    with Engine(0) as F:
+        fresh, _, _ = F.batch_exact(small)
+        sfresh, sfres, _, _ = F.batch_sampled(ssmall)
+    assert [(a.re, a.im, a.terms, a.ips) for a in after] == [(b.re, b.im, b.terms, b.ips) for b in fresh]
+    assert np.array_equal(safter, sfresh)
+    assert [(a.ips, a.vanished) for a in sres] == [(b.ips, b.vanished) for b in sfres]
+
+
 @pytest.mark.parametrize("bins", [1, 3, 4])
 def test_projector_norm_bins_in_one_call(E, bins):
     # cse_projector_norm sends every bin as one instance of one batched call;
