@@ -35,13 +35,22 @@ struct cse_ctx {
     int cus = 0;
     char name[128] = {0};
     char err[512] = {0};
-    DevBuf s1, s2, out, recs, terms, zsum, kahan, hist, ips, queue, thr, sexp, L, idx, errflag, lsh;
+    DevBuf s1, s2, out, recs, terms, queue, thr, sexp, L, idx, errflag, lsh;
     // a batch's plan: instance descriptors, item and record offsets and
     // projectors in one allocation, laid out as the pinned staging buffer
     DevBuf plan;
     InstDesc *d_inst = nullptr;
     long long *d_itemoff = nullptr, *d_recoff = nullptr;
     DevProjector *d_projs = nullptr;
+    // a batch's results (sums, Kahan pairs, histograms, counts, error flag) in
+    // one allocation, laid out as the pinned stage they are copied into
+    DevBuf outb;
+    size_t out_bytes = 0;
+    long long *d_zsum = nullptr;
+    double *d_kahan = nullptr;
+    unsigned long long *d_hist = nullptr, *d_ips = nullptr;
+    int32_t *d_sexp = nullptr;
+    int *d_err = nullptr;
     HostBuf hstage;  // results coming back (per-sample sums, histograms)
     HostBuf hup;     // the plan going up
     bool thr_ready = false;
@@ -244,15 +253,6 @@ int upload_plan(cse_ctx *ctx, const Plan &pl) {
     return CSE_OK;
 }
 
-// device buffers of the record launch; its error flag joins the batch's clear
-int prep_records(cse_ctx *ctx, const Plan &pl, ZeroList &zl) {
-    ENSURE(recs, (size_t)REC_WORDS * 8 * (size_t)pl.nrec);
-    ENSURE(sexp, 4 * (size_t)pl.nrec);
-    ENSURE(errflag, 8);
-    zl_add(zl, ctx->errflag.p, 1);
-    return CSE_OK;
-}
-
 int run_records(cse_ctx *ctx, const Plan &pl, int mode) {
     const int ninst = (int)pl.inst.size();
     const long long waves = pl.nrec;
@@ -260,45 +260,66 @@ int run_records(cse_ctx *ctx, const Plan &pl, int mode) {
     if (blocks > 0) {
         hipLaunchKernelGGL(k_records, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, mode, ninst,
                            ctx->d_recoff, ctx->d_inst, ctx->d_projs,
-                           dp<uint64_t>(ctx->thr), pl.nrec, dp<uint64_t>(ctx->recs),
-                           dp<int32_t>(ctx->sexp), dp<int>(ctx->errflag));
+                           dp<uint64_t>(ctx->thr), pl.nrec, dp<uint64_t>(ctx->recs), ctx->d_sexp,
+                           ctx->d_err);
         HIPCHK(hipGetLastError());
     }
     return CSE_OK;
 }
 
-// device buffers of the sweep; its sums, counters and queue join the
-// batch's clear
-int prep_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms, ZeroList &zl) {
-    const int ninst = (int)pl.inst.size();
-    ENSURE(queue, 8);
-    ENSURE(ips, 8 * (size_t)ninst);
-    // the histograms are those of exact mode; sampled mode sums per record
-    if (mode == 1) ENSURE(hist, 8 * (size_t)HIST_STRIDE * ninst);
+// Layout of a batch's results block, the same on the device and in the
+// pinned stage.  Sampled mode (0): per-sample Z[omega] sums (32 B), Kahan
+// pairs (16 B) and projection exponents (4 B), then per-instance counts and
+// the error flag.  Exact mode (1): per-instance histograms, Kahan pairs and
+// counts, then the error flag.
+struct OutLayout {
+    size_t zsum, kahan, sexp, hist, ips, err, total;
+};
+
+OutLayout out_layout(const Plan &pl, int mode) {
+    const size_t n = (size_t)pl.nrec, ni = pl.inst.size();
+    OutLayout o;
+    memset(&o, 0, sizeof(o));
     if (mode == 0) {
-        ENSURE(zsum, 32 * (size_t)pl.nrec);
-        ENSURE(kahan, 16 * (size_t)pl.nrec);
-        zl_add(zl, ctx->zsum.p, 4 * pl.nrec);
-        zl_add(zl, ctx->kahan.p, 2 * pl.nrec);
+        o.zsum = 0;
+        o.kahan = 32 * n;
+        o.sexp = 48 * n;
+        o.ips = (52 * n + 7) & ~(size_t)7;
+        o.err = o.ips + 8 * ni;
     } else {
-        ENSURE(kahan, 16 * (size_t)ninst);
-        zl_add(zl, ctx->kahan.p, 2 * (long long)ninst);
+        const size_t nh = (size_t)HIST_STRIDE * ni;
+        o.hist = 0;
+        o.kahan = 8 * nh;
+        o.ips = o.kahan + 16 * ni;
+        o.err = o.ips + 8 * ni;
     }
-    zl_add(zl, ctx->queue.p, 1);
-    zl_add(zl, ctx->ips.p, ninst);
-    if (mode == 1) zl_add(zl, ctx->hist.p, (long long)HIST_STRIDE * ninst);
-    if (want_terms) ENSURE(terms, 4 * (size_t)(pl.nterms > 0 ? pl.nterms : 1));
-    return CSE_OK;
+    o.total = o.err + 8;
+    return o;
 }
 
-// the batch's buffers, cleared in one launch: prep_records, then prep_sweep
+// The batch's device buffers, and one launch that clears its results block
+// (every sum, count and flag; the exponents are rewritten by the record
+// launch) and its work queue.
 int prep_batch(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
+    const OutLayout o = out_layout(pl, mode);
+    ENSURE(outb, o.total);
+    ENSURE(recs, (size_t)REC_WORDS * 8 * (size_t)pl.nrec);
+    ENSURE(queue, 8);
+    if (mode == 1) ENSURE(sexp, 4 * (size_t)pl.nrec);
+    if (want_terms) ENSURE(terms, 4 * (size_t)(pl.nterms > 0 ? pl.nterms : 1));
+    char *b = (char *)ctx->outb.p;
+    ctx->out_bytes = o.total;
+    ctx->d_zsum = mode == 0 ? (long long *)(b + o.zsum) : nullptr;
+    ctx->d_kahan = (double *)(b + o.kahan);
+    ctx->d_sexp = mode == 0 ? (int32_t *)(b + o.sexp) : dp<int32_t>(ctx->sexp);
+    ctx->d_hist = mode == 1 ? (unsigned long long *)(b + o.hist) : nullptr;
+    ctx->d_ips = (unsigned long long *)(b + o.ips);
+    ctx->d_err = (int *)(b + o.err);
     ZeroList zl;
     memset(&zl, 0, sizeof(zl));
-    int rc = prep_records(ctx, pl, zl);
-    if (!rc) rc = prep_sweep(ctx, pl, mode, want_terms, zl);
-    if (!rc) rc = zero_list(ctx, zl);
-    return rc;
+    zl_add(zl, ctx->outb.p, (long long)(o.total / 8));
+    zl_add(zl, ctx->queue.p, 1);
+    return zero_list(ctx, zl);
 }
 
 int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
@@ -312,12 +333,12 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     A.recs = dp<uint64_t>(ctx->recs);
     A.queue = dp<unsigned long long>(ctx->queue);
     A.terms = want_terms ? dp<int32_t>(ctx->terms) : nullptr;
-    A.zsum = mode == 0 ? dp<long long>(ctx->zsum) : nullptr;
-    A.kahan = dp<double>(ctx->kahan);
-    A.hist = mode == 1 ? dp<unsigned long long>(ctx->hist) : nullptr;
+    A.zsum = ctx->d_zsum;
+    A.kahan = ctx->d_kahan;
+    A.hist = ctx->d_hist;
     A.hist_stride = HIST_STRIDE;
-    A.ips = dp<unsigned long long>(ctx->ips);
-    A.err = dp<int>(ctx->errflag);
+    A.ips = ctx->d_ips;
+    A.err = ctx->d_err;
     bool packed = true;
     for (const InstDesc &d : pl.inst) packed = packed && d.n <= CSE_END_ROW;
     // exact mode takes k_sweep_packed_all (seven waves per SIMD); sampled
@@ -430,9 +451,8 @@ void cse_destroy(cse_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->s1, &ctx->s2, &ctx->out, &ctx->recs, &ctx->terms, &ctx->zsum, &ctx->kahan,
-                      &ctx->hist, &ctx->ips, &ctx->queue, &ctx->thr, &ctx->sexp, &ctx->L, &ctx->idx,
-                      &ctx->errflag, &ctx->lsh, &ctx->plan};
+    DevBuf *bufs[] = {&ctx->s1, &ctx->s2, &ctx->out, &ctx->recs, &ctx->terms, &ctx->queue, &ctx->thr,
+                      &ctx->sexp, &ctx->L, &ctx->idx, &ctx->errflag, &ctx->lsh, &ctx->plan, &ctx->outb};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (ctx->hstage.p) (void)hipHostFree(ctx->hstage.p);
@@ -597,22 +617,17 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
 
-    // one pinned block: z (32 B/sample), kz (16), sexp (4), then the
-    // per-instance inner-product counts and the error flag
-    const size_t n = (size_t)pl.nrec, ni = (size_t)ninst, oips = (52 * n + 7) & ~(size_t)7;
-    rc = ensure_host(ctx, ctx->hstage, oips + 8 * ni + 8);
+    // the results block in one copy, into a pinned stage of the same layout
+    const OutLayout o = out_layout(pl, 0);
+    rc = ensure_host(ctx, ctx->hstage, o.total);
     if (rc) return rc;
     char *hs = (char *)ctx->hstage.p;
-    const long long *z = (const long long *)hs;
-    const double *kz = (const double *)(hs + 32 * n);
-    const int32_t *se = (const int32_t *)(hs + 48 * n);
-    const unsigned long long *nips = (const unsigned long long *)(hs + oips);
-    int *errp = (int *)(hs + oips + 8 * ni);
-    HIPCHK(hipMemcpyAsync(hs, ctx->zsum.p, 32 * n, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(hs + 32 * n, ctx->kahan.p, 16 * n, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(hs + 48 * n, ctx->sexp.p, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(hs + oips, ctx->ips.p, 8 * ni, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(errp, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    const long long *z = (const long long *)(hs + o.zsum);
+    const double *kz = (const double *)(hs + o.kahan);
+    const int32_t *se = (const int32_t *)(hs + o.sexp);
+    const unsigned long long *nips = (const unsigned long long *)(hs + o.ips);
+    int *errp = (int *)(hs + o.err);
+    HIPCHK(hipMemcpyAsync(hs, ctx->outb.p, o.total, hipMemcpyDeviceToHost, ctx->stream));
     for (int i = 0; i < ninst; i++)
         if (res[i].terms)
             HIPCHK(hipMemcpyAsync(res[i].terms, dp<int32_t>(ctx->terms) + pl.inst[i].term_off,
@@ -727,20 +742,18 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
     rc = run_sweep(ctx, pl, 1, terms_out != nullptr);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
-    // one pinned block: histograms (HIST_STRIDE words per instance), the
-    // Kahan pairs, the inner-product counts, the error flag
-    const size_t ni = (size_t)ninst, nh = (size_t)HIST_STRIDE * ni;
-    rc = ensure_host(ctx, ctx->hstage, 8 * nh + 16 * ni + 8 * ni + 8);
+    // the results block in one copy, into a pinned stage of the same layout:
+    // histograms (HIST_STRIDE words per instance), the Kahan pairs, the
+    // inner-product counts, the error flag
+    const OutLayout o = out_layout(pl, 1);
+    rc = ensure_host(ctx, ctx->hstage, o.total);
     if (rc) return rc;
     char *hs = (char *)ctx->hstage.p;
-    const unsigned long long *hist = (const unsigned long long *)hs;
-    const double *kz = (const double *)(hs + 8 * nh);
-    const unsigned long long *nips = (const unsigned long long *)(hs + 8 * nh + 16 * ni);
-    int *errp = (int *)(hs + 8 * nh + 24 * ni);
-    HIPCHK(hipMemcpyAsync(errp, ctx->errflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(hs, ctx->hist.p, 8 * nh, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(hs + 8 * nh, ctx->kahan.p, 16 * ni, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(hs + 8 * nh + 16 * ni, ctx->ips.p, 8 * ni, hipMemcpyDeviceToHost, ctx->stream));
+    const unsigned long long *hist = (const unsigned long long *)(hs + o.hist);
+    const double *kz = (const double *)(hs + o.kahan);
+    const unsigned long long *nips = (const unsigned long long *)(hs + o.ips);
+    int *errp = (int *)(hs + o.err);
+    HIPCHK(hipMemcpyAsync(hs, ctx->outb.p, o.total, hipMemcpyDeviceToHost, ctx->stream));
     if (terms_out)
         HIPCHK(hipMemcpyAsync(terms_out, ctx->terms.p, 4 * (size_t)pl.nterms, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
