@@ -295,8 +295,9 @@ def shots_config(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 3; c1: 200 probability evaluations of about 0.5 ms each)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1; c1: 20)")
     ap.add_argument("--samples", type=int, default=65536, help="random thetas per step (M)")
     ap.add_argument("--chunk", default=None,
                     help="item size in j, 0 for the engine's rule, or auto for the autotuner's choice "
@@ -314,6 +315,10 @@ def main():
                     help="optimisation level of the reference C build the CPU baseline times")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: cores, at most 16)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 200 if args.config == "c1" else 3
+    if args.warmup is None:
+        args.warmup = 20 if args.config == "c1" else 1
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this engine is measured on one GPU; multi-GPU runs are not part of this build "

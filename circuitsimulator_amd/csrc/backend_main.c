@@ -86,17 +86,24 @@ static int mpi_rank(void)
 
 static cse_ctx *g_ctx = NULL;
 
-static double projector_norm(const cse_projector *P, const uint64_t *L, int k, int exact, double norm,
-                             int sampled, int samples, int bins, uint64_t seed)
+/* ||G'|H^t>||^2 and ||H'|H^t>||^2 in one engine call (probability.c:185-205) */
+static void projector_norms(const cse_projector *G, const cse_projector *H, const uint64_t *L, int k, int exact,
+                            double norm, int sampled, int samples, int bins, uint64_t seed, double *num,
+                            double *den)
 {
     int dev = 0;
     const char *s = getenv("CSE_DEVICE");
     if (s && *s) dev = atoi(s);
-    double v = 0;
-    const int rc = cse_projector_norm(&g_ctx, dev, P, L, k, exact, norm, sampled, samples, bins, seed, &v);
+    cse_projector P[2];
+    P[0] = *G;
+    P[1] = *H;
+    const uint64_t seeds[2] = {seed, seed ^ 0x9E3779B97F4A7C15ull};
+    double v[2] = {0, 0};
+    const int rc = cse_projector_norms(&g_ctx, dev, 2, P, seeds, L, k, exact, norm, sampled, samples, bins, v);
     if (rc == CSE_ERR_NODEV) die(4, "no usable gfx950 GPU for the engine", g_ctx ? cse_error(g_ctx) : NULL);
     if (rc != CSE_OK) die(5, "engine failure:", g_ctx ? cse_error(g_ctx) : NULL);
-    return v;
+    *num = v[0];
+    *den = v[1];
 }
 
 int main(int argc, char *argv[])
@@ -190,9 +197,8 @@ int main(int argc, char *argv[])
     }
 
     const int sampled = noapprox == 0;
-    const double numerator = projector_norm(&G, L, k, exact, norm, sampled, samples, bins, seed);
-    const double denominator =
-        projector_norm(&H, L, k, exact, norm, sampled, samples, bins, seed ^ 0x9E3779B97F4A7C15ull);
+    double numerator = 0, denominator = 0;
+    projector_norms(&G, &H, L, k, exact, norm, sampled, samples, bins, seed, &numerator, &denominator);
 
     printf("|| Gprime |H^t> ||^2 ~= %.17e\n", numerator);
     printf("|| Hprime |H^t> ||^2 ~= %.17e\n", denominator);

@@ -108,25 +108,32 @@ int cse_sampling_kept(int noapprox, int forceSample, int exact, int t, int k, in
     return (double)samples * (double)bins * 2 > lim ? 0 : 1;
 }
 
-int cse_projector_norm(cse_ctx **ctx, int device, const cse_projector *P, const uint64_t *L, int k, int exact,
-                       double norm, int sampled, int samples, int bins, uint64_t seed, double *value)
+/* the cases that need no engine: 1 with *value set, else 0 */
+static int closed_form(const cse_projector *P, double norm, int sampled, int samples, int bins, double *value)
 {
-    if (!ctx || !P || !value) return CSE_ERR_ARG;
     if (P->nstabs == 0) {
         *value = norm * norm;
-        return CSE_OK;
+        return 1;
     }
     if (P->nqubits == 0) {
         /* the sampled form keeps the norm factor, the exact one does not */
         const double f = clifford_sum(P) / (1 + (double)P->nstabs);
         *value = sampled ? norm * norm * f : f;
-        return CSE_OK;
+        return 1;
     }
     if (sampled && (samples <= 0 || bins <= 0)) {
         /* the reference divides 0 by 0 here (innerprod.c:83) */
         *value = NAN;
-        return CSE_OK;
+        return 1;
     }
+    return 0;
+}
+
+int cse_projector_norm(cse_ctx **ctx, int device, const cse_projector *P, const uint64_t *L, int k, int exact,
+                       double norm, int sampled, int samples, int bins, uint64_t seed, double *value)
+{
+    if (!ctx || !P || !value) return CSE_ERR_ARG;
+    if (closed_form(P, norm, sampled, samples, bins, value)) return CSE_OK;
     int rc = ensure_ctx(ctx, device);
     if (rc != CSE_OK) return rc;
     if (!sampled) {
@@ -144,5 +151,52 @@ int cse_projector_norm(cse_ctx **ctx, int device, const cse_projector *P, const 
         *value = (bins % 2) ? vals[(bins - 1) / 2] : (vals[bins / 2] + vals[bins / 2 - 1]) / 2;
     }
     free(vals);
+    return rc;
+}
+
+int cse_projector_norms(cse_ctx **ctx, int device, int nproj, const cse_projector *P, const uint64_t *seeds,
+                        const uint64_t *L, int k, int exact, double norm, int sampled, int samples, int bins,
+                        double *values)
+{
+    if (!ctx || !P || !values || nproj < 0 || (sampled && nproj > 0 && !seeds)) return CSE_ERR_ARG;
+    if (sampled) {
+        for (int i = 0; i < nproj; i++) {
+            const int rc = cse_projector_norm(ctx, device, &P[i], L, k, exact, norm, sampled, samples, bins,
+                                              seeds[i], &values[i]);
+            if (rc != CSE_OK) return rc;
+        }
+        return CSE_OK;
+    }
+    /* exact pair sums: every projector the closed forms leave is one
+       instance of one batched launch */
+    int *which = (int *)malloc(sizeof(int) * (size_t)(nproj > 0 ? nproj : 1));
+    if (!which) return CSE_ERR_NOMEM;
+    int m = 0;
+    for (int i = 0; i < nproj; i++)
+        if (!closed_form(&P[i], norm, 0, samples, bins, &values[i])) which[m++] = i;
+    int rc = CSE_OK;
+    if (m > 0) rc = ensure_ctx(ctx, device);
+    cse_instance *in = NULL;
+    cse_exact_result *res = NULL;
+    if (rc == CSE_OK && m > 0) {
+        in = (cse_instance *)calloc((size_t)m, sizeof(cse_instance));
+        res = (cse_exact_result *)calloc((size_t)m, sizeof(cse_exact_result));
+        if (!in || !res) rc = CSE_ERR_NOMEM;
+    }
+    if (rc == CSE_OK && m > 0) {
+        for (int j = 0; j < m; j++) {
+            in[j].P = P[which[j]];
+            in[j].exact = exact ? 1 : 0;
+            in[j].k = exact ? 0 : k;
+            if (!exact)
+                for (int r = 0; r < k; r++) in[j].L[r] = L[r];
+        }
+        rc = cse_batch_exact(*ctx, in, m, 0, res, NULL, NULL);
+        if (rc == CSE_OK)
+            for (int j = 0; j < m; j++) values[which[j]] = res[j].value;
+    }
+    free(in);
+    free(res);
+    free(which);
     return rc;
 }

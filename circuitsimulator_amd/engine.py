@@ -24,7 +24,7 @@ P = ctypes.POINTER
 ABI_SYMBOLS = [
     "cse_create", "cse_destroy", "cse_error", "cse_device_info", "cse_inner_products",
     "cse_prep_states", "cse_random_states", "cse_exact_projector", "cse_sampled_projector",
-    "cse_batch_exact", "cse_decompose", "cse_hist_value", "cse_sampling_kept", "cse_projector_norm",
+    "cse_batch_exact", "cse_decompose", "cse_hist_value", "cse_sampling_kept", "cse_projector_norm", "cse_projector_norms",
     "cse_lspace_weights", "cse_batch_sampled", "cse_decompose_many",
 ]
 ERR_NODEV = -5
@@ -130,6 +130,9 @@ def lib():
         L.cse_projector_norm.argtypes = [P(ctypes.c_void_p), ctypes.c_int, P(Projector), P(U64), ctypes.c_int,
                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          U64, P(ctypes.c_double)]
+        L.cse_projector_norms.argtypes = [P(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, P(Projector), P(U64),
+                                          P(U64), ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -366,6 +369,24 @@ def projector_norm(proj, L, k, exact, norm, sampled, samples=0, bins=1, seed=0, 
     if rc != 0:
         raise EngineError(f"engine error {rc}: {lib().cse_error(h).decode()}")
     return v.value
+
+
+def projector_norms_batch(projs, seeds, L, k, exact, norm, sampled, samples=0, bins=1, device=0):
+    """[projector_norm(projs[i], ..., seeds[i]) ...] in one cse_projector_norms
+    call: exact pair sums of every projector in one batched launch."""
+    h = _norm_ctx.setdefault(device, ctypes.c_void_p())
+    n = len(projs)
+    arr = (Projector * n)(*projs)
+    sd = (U64 * max(1, n))(*[s & (2 ** 64 - 1) for s in seeds])
+    Larr = (U64 * NMAX)(*(list(L) + [0] * (NMAX - len(L))))
+    vals = (ctypes.c_double * max(1, n))()
+    rc = lib().cse_projector_norms(ctypes.byref(h), device, n, arr, sd, Larr, int(k), int(bool(exact)),
+                                   float(norm), int(bool(sampled)), int(samples), int(bins), vals)
+    if rc == ERR_NODEV:
+        raise EngineError(f"no usable gfx950 device {device} for the engine")
+    if rc != 0:
+        raise EngineError(f"engine error {rc}: {lib().cse_error(h).decode()}")
+    return [vals[i] for i in range(n)]
 
 
 def hist_value(hist, pmin=-128):

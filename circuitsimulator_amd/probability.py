@@ -12,7 +12,7 @@ reference's backend computes them (probability.c:153-216): decompose picks
 |H^t> exactly (chi = 2^ceil(t/2)) or an |L> of rank k, the sampling-disable
 rule may switch to exact evaluation, and each norm is exact or a median of
 means of sampled estimates -- all through cse_decompose,
-cse_sampling_kept and cse_projector_norm of libcse.
+cse_sampling_kept and cse_projector_norms of libcse.
 
 config keys are the reference's (see libcirc/probability.py:14-65):
 verbose, silenceprojectors, quiet, direct, noapprox, samples, bins, error,
@@ -30,7 +30,7 @@ width from t instead of raising IndexError (probability.py:259).
 import numpy as np
 
 from .compile import projectors as _projectors
-from .engine import Projector, lib, projector_norm, sampling_kept
+from .engine import Projector, lib, projector_norm, projector_norms_batch, sampling_kept
 
 _SEED_MASK = (1 << 64) - 1
 
@@ -167,9 +167,10 @@ def projector_norms(Gprime, Hprime, t, config):
     if not config.get("noapprox") and not sampled and verbose:
         print("More samples than terms in exact calculation. Disabling sampling.")
     device = int(config.get("device") or 0)
-    num = projector_norm(_engine_projector(Gprime, t), L, k, exact, norm, sampled, samples, bins, seed, device)
-    den = projector_norm(_engine_projector(Hprime, t), L, k, exact, norm, sampled, samples, bins,
-                         seed ^ 0x9E3779B97F4A7C15, device)
+    # both norms in one call (one batched launch for exact pair sums)
+    num, den = projector_norms_batch([_engine_projector(Gprime, t), _engine_projector(Hprime, t)],
+                                     [seed, seed ^ 0x9E3779B97F4A7C15], L, k, exact, norm, sampled, samples,
+                                     bins, device)
     return num, den
 
 

@@ -236,6 +236,19 @@ int cse_sampling_kept(int noapprox, int forceSample, int exact, int t, int k, in
 int cse_projector_norm(cse_ctx **ctx, int device, const cse_projector *P, const uint64_t *L, int k, int exact,
                        double norm, int sampled, int samples, int bins, uint64_t seed, double *value);
 
+/*
+ * cse_projector_norm for nproj projectors of one decomposition (G' and H' of
+ * probability.c:185-205, whose two exactProjector / multiSampledProjector
+ * calls it replaces): values[i] is cse_projector_norm of P[i] with seed
+ * seeds[i].  With sampled == 0 every projector the closed forms leave is one
+ * instance of a single cse_batch_exact launch (the values are exact sums, so
+ * they equal the single calls' bit for bit); sampled projectors run one
+ * after another.  seeds may be NULL when sampled == 0.
+ */
+int cse_projector_norms(cse_ctx **ctx, int device, int nproj, const cse_projector *P, const uint64_t *seeds,
+                        const uint64_t *L, int k, int exact, double norm, int sampled, int samples, int bins,
+                        double *values);
+
 #ifdef __cplusplus
 }
 #endif

@@ -373,6 +373,29 @@ def test_state_calls_reject_counts_above_the_limit(E):
     assert res.shape == (2, 16)
 
 
+def test_projector_norms_batch_equals_single_calls():
+    # cse_projector_norms: exact pair sums of several projectors in one
+    # launch equal cse_projector_norm's single calls bit for bit; closed
+    # forms and sampled projectors (own seeds) as the single calls
+    from circuitsimulator_amd.engine import Projector, projector_norm, projector_norms_batch
+    from circuitsimulator_amd.workloads import sampled_config
+    P1, _, L = sampled_config(14, 5, 6, seed=71)
+    P2, _, _ = sampled_config(14, 4, 6, seed=72)
+    empty = Projector()
+    empty.nqubits = 14
+    norm = 1.7
+    for exact, k in [(0, 6), (1, 0)]:
+        projs = [P1, P2, empty, P1]
+        seeds = [11, 12, 13, 14]
+        got = projector_norms_batch(projs, seeds, L, k, exact, norm, False)
+        want = [projector_norm(p, L, k, exact, norm, False, seed=sd) for p, sd in zip(projs, seeds)]
+        assert got == want, (exact, got, want)
+        assert got[2] == norm * norm and got[0] == got[3]
+    got = projector_norms_batch([P1, P2], [5, 6], L, 6, 0, norm, True, samples=64, bins=3)
+    want = [projector_norm(P1, L, 6, 0, norm, True, 64, 3, 5), projector_norm(P2, L, 6, 0, norm, True, 64, 3, 6)]
+    assert got == want
+
+
 def test_reused_buffers_start_clean(E):
     # a large batch, then a smaller one on the same context: the one clear
     # launch resets every sum, counter, queue and flag the smaller call
