@@ -167,6 +167,39 @@ def exact_pairs_config(args):
     }
 
 
+def sampled_h_config(args):
+    """Sampled mode with the |H^t> decomposition (probability()'s default
+    decomposition; innerprod.c:88-144 with prepH, stateprep.c:36-81): t=40,
+    r=12, 512 random states x chi = 2^20 terms <prepH(j)|theta'> each.
+    value = inner products/s of the whole call."""
+    from circuitsimulator_amd import Engine
+    from circuitsimulator_amd.workloads import sampled_config
+    t, r, M = 40, 12, 512
+    P, _, _ = sampled_config(t, r, 1, seed=1040)
+    chi = 1 << ((t + 1) // 2)
+    with Engine(0) as E:
+        for w in range(args.warmup):
+            E.sampled_projector(P, [], 0, 1, 7, 0, 64, want_samples=False)
+        t0 = time.perf_counter()
+        kernel_ms = 0.0
+        ips = 0
+        for _ in range(args.steps):
+            res, _ = E.sampled_projector(P, [], 0, 1, 7, 0, M, want_samples=False)
+            kernel_ms += res.kernel_ms
+            ips += res.ips
+        dt = time.perf_counter() - t0
+    return {
+        "metric": "sampled-mode inner products/s with the |H^t> decomposition, t=40 r=12",
+        "value": ips / dt, "unit": "inner products/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1000.0 / args.steps, "higher_is_better": True,
+        "dtype": "int64", "data": "synthetic (seeded Philox random stabilizer states; seeded commuting Pauli group)",
+        "config": {"workload": f"sampled |H^t>: t={t} r={r}, {M} random states x chi=2^{(t + 1) // 2}",
+                   "t": t, "r": r, "samples": M, "chi": chi, "seed": 1040},
+        "kernel_ip_per_s": ips / (kernel_ms / 1000.0),
+        "result": {"mean_int64_exact": res.mean, "mean_kahan_fp32": res.mean_kahan, "vanished": res.vanished},
+    }
+
+
 def exact_h_config(args):
     """The exact projector with the |H^t> decomposition (probability()'s
     default, innerprod.c:148-261 with prepH, stateprep.c:36-81): t=30, r=10,
@@ -304,9 +337,10 @@ def main():
                          "(default: auto for c2 and c3, 0 otherwise)")
     ap.add_argument("--plain-pairs", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5", "h30"], default="c4",
+    ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5", "h30", "hs40"], default="c4",
                     help="c4: the headline norm estimation (default); c1, c2, c3, c5: the other configurations; "
-                         "h30: the exact projector with the |H^t> decomposition at t=30")
+                         "h30: the exact projector with the |H^t> decomposition at t=30; hs40: sampled mode "
+                         "with the |H^t> decomposition at t=40")
     ap.add_argument("--shots", type=int, default=1024, help="c5: number of shots")
     ap.add_argument("--instances", type=int, default=4096, help="c3: number of instances")
     ap.add_argument("--cpu-baseline-only", action="store_true",
@@ -327,7 +361,7 @@ def main():
 
     if args.config != "c4":
         out = {"c1": circuit_config, "c2": exact_pairs_config, "c3": batch_config,
-               "c5": shots_config, "h30": exact_h_config}[args.config](args)
+               "c5": shots_config, "h30": exact_h_config, "hs40": sampled_h_config}[args.config](args)
         print(json.dumps(out), flush=True)
         return
 
