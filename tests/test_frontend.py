@@ -160,6 +160,71 @@ def test_imports_and_comments(libdir):
         compilecirc.compileCircuit(fname=str(libdir / "a.circ"))  # import loop
 
 
+def test_ancilla_wires_in_call_order():
+    # wires past main's qubits go out in call order: a handed-back ('R')
+    # position takes its wire at the symbol's first call and keeps it, every
+    # other ancilla position takes a fresh wire per call; a call under 'M'
+    # marks every gate it expands to (expected rows derived by hand)
+    circ = compilecirc.compileCircuit(raw=LIB + "main:\nW\nQ\nW\nQ\n")
+    assert circ.splitlines() == ["CX__", "CX__",
+                                 "C_X_", "__H_", "H_M_", "X_M_", "H_M_",
+                                 "CX__", "CX__",
+                                 "C__X", "___H", "H__M", "X__M", "H__M"]
+    # nested: AB's handed-back position 3 gets wire 2 (first call) before its
+    # fresh position 2 (wire 3); Q inside takes wire 4, W wire 5; the second
+    # call "BA" passes its arguments swapped and gets fresh wires 6, 7
+    circ = compilecirc.compileCircuit(raw=LIB + "AB:\n__Q_\nC_X_\n___W\n___R\nmain:\nAB\nBA\n")
+    assert circ.splitlines() == ["___CX___", "____H___", "___HM___", "___XM___", "___HM___",
+                                 "C__X____", "__C__X__", "__C__X__",
+                                 "______CX", "_______H", "______HM", "______XM", "______HM",
+                                 "_C____X_", "__C__X__", "__C__X__"]
+
+
+def test_language_errors_name_the_problem(tmp_path):
+    cases = [
+        ("main:\nH\n/* a */\n", "Error at ./<raw input>:3 - Unexpected '*/'."),
+        ("main:\n/*\n/* x\n*/\nH\n", "Error at ./<raw input>:3 - Unexpected '/*' inside block comment."),
+        ("main:\nH\n/*\n", "Error at ./<raw input>:3 - Expected '*/' before end of file."),
+        ("H\n", "Error at ./<raw input>:1 - Expected function declaration."),
+        ("main:\nH\nHH\n", "Error at ./<raw input>:3 - Bad number of qubits (expected 1)."),
+        ("BA:\nX\n", "Error at ./<raw input>:1 - Symbol BA is not in standard form (expected AB)."),
+        ("CX:\nX\n", "Error at ./<raw input>:1 - Symbol CX is in gate set."),
+        ("main:\nH\nmain:\nX\n", "Error at ./<raw input>:3 - Symbol main already defined."),
+        ("main:\nHQ\n", "Undefined gate: HQ"),
+        ("AB:\nCD\nCD:\nAB\nmain:\nAB\n", "Dependency loop: ['CD', 'AB'] -> CD"),
+        ("main:\nH_M\n", "In main: cannot measure non-ancilla qubit on line H_M"),
+        ("Z:\nH_\nHM\n_H\nmain:\nZ\n", "In Z: cannot perform operation on measured qubit 1 on line _H"),
+        ("Z:\nR_\nmain:\nZ\n", "In Z: cannot reuse non-ancilla qubit on line R_"),
+        ("Z:\n_R\n_H\nmain:\nZ\n", "In Z: cannot perform operation on reused qubit 1 on line _H"),
+        ("frm x import y\n", "Error at ./<raw input>:1 - Invalid import statement."),
+        ("main:\nH\n", None),
+    ]
+    for text, msg in cases:
+        if msg is None:
+            assert compilecirc.compileCircuit(raw=text) == "H\n"
+            continue
+        with pytest.raises(SyntaxError) as e:
+            compilecirc.compileCircuit(raw=text)
+        assert str(e.value) == msg, text
+    with pytest.raises(SyntaxError, match="Could not find symbol AB to compile"):
+        compilecirc.compileCircuit(raw="main:\nH\n", name="AB")
+    (tmp_path / "lib.circ").write_text(LIB)
+    f = tmp_path / "m.circ"
+    f.write_text("from lib.circ import Q Nope\nmain:\nQ\n")
+    with pytest.raises(SyntaxError, match="Failed to import symbol\\(s\\): 'Nope'"):
+        compilecirc.compileCircuit(fname=str(f))
+    f.write_text("Q:\nH\nfrom lib.circ import Q\nmain:\nQ\n")
+    with pytest.raises(SyntaxError, match="Can't import Q because it is already defined!"):
+        compilecirc.compileCircuit(fname=str(f))
+    f.write_text("import nothere.circ\n")
+    with pytest.raises(SyntaxError, match="File .*nothere.circ not found."):
+        compilecirc.compileCircuit(fname=str(f))
+    # an imported symbol brings every symbol it needs along, at any depth
+    (tmp_path / "deep.circ").write_text("AB:\nCZ\nCZ:\n_H\nCX\n_H\n")
+    f.write_text("from deep.circ import AB\nmain:\nAB\n")
+    assert compilecirc.compileCircuit(fname=str(f)).splitlines() == ["_H", "CX", "_H"]
+
+
 def test_countY_rejects_gates_on_measured_qubits():
     with pytest.raises(SyntaxError):
         gadgetize.countY("_H\nCX\nHM\nH_\n_H\n")
