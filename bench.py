@@ -20,6 +20,20 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# Algorithmic work per inner product of the elimination algorithm the sweep
+# runs (fixation of the pivot coordinates outside supp(x_j), restrictions on
+# non-pivot columns, A/B/C elimination steps), in 64-bit word operations of a
+# dense-row formulation: the mean over 64 projected random states x 512
+# seeded j of this workload, from the register-level model of the kernel
+# (tests/test_sweep_model.py, algorithmic_work), which checks it.  Two int32
+# lane operations per word operation.
+ALG_WORDOPS_PER_IP = 982.0
+ALG_LANE_OPS_PER_IP = 2 * ALG_WORDOPS_PER_IP
+# The canonical dense-row algorithm's count (the reference's shrink /
+# basis-alignment / exponential-sum sequence, oracle/cse_oracle.c
+# orc_ip_wordops), mean over the same workload
+CANONICAL_WORDOPS_PER_IP = 14743
+
 # Per inner product of the sweep kernel on this workload, from rocprofv3
 # --pmc over one dispatch of 2^28 inner products
 # (profiles/k_sweep_norm_estimation_t48_r24_k12_M65536.md):
@@ -50,41 +64,52 @@ def exact_mean(t, sexp, zsum):
     return float(tot / len(sexp))
 
 
-def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0, lib="libref.so", threads=0):
-    """The reference's own C (oracle/_ref/libref.so, compiled from the reference
-    sources at -O2; libref_O0.so at -O0) on a bounded sample of the same
-    workload: the first nphi decomposition states for `threads` projected
-    thetas (default: the host's cores, at most 16), one POSIX thread per theta.
-    Falls back to the oracle (a port) when the library is absent."""
+def baseline_thetas(P_tuple, t, seed, count):
+    """The first `count` non-vanishing projected thetas of the Philox stream
+    keyed by seed (the draws of timed step 0): (sample indices, oracle
+    states, projection exponents, the first index past them)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import ctypes
-
     import orc
     ph, xs, zs = P_tuple
     Po = orc.make_projector(ph, xs, zs, t)
     thr = orc.thresholds(t)
-    threads = threads or max(1, min(16, os.cpu_count() or 1))
-    thetas = []
+    idx, states, sexp = [], [], []
     st = 0
-    while len(thetas) < threads:
+    while len(states) < count:
         th = orc.random_state(t, seed, st, thr)
+        se = orc.project(th, Po)
+        if se >= 0:
+            idx.append(st)
+            states.append(th)
+            sexp.append(se)
         st += 1
-        if orc.project(th, Po) >= 0:
-            thetas.append(th)
-    Larr = (orc.U64 * k)(*L)
+    return idx, states, sexp, st
+
+
+def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0, lib="libref.so", threads=0, detail=None):
+    """The reference's own C (oracle/_ref/libref.so, compiled from the reference
+    sources at -O2; libref_O0.so at -O0) on a bounded sample of the same
+    workload: the first nphi decomposition states for `threads` projected
+    thetas (default: the host's cores, at most 16), one POSIX thread per theta.
+    Falls back to the oracle (a port) when the library is absent.  detail (a
+    dict) receives the sample: theta indices and states, nphi, the reference's
+    per-term results and per-theta partial sums."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    idx, thetas, sexp, _ = baseline_thetas(P_tuple, t, seed, threads)
+    if detail is not None:
+        detail.update({"index": idx, "thetas": thetas, "sexp": sexp})
     if orc.ref_available(lib):
-        R = orc.ref(lib)
-        arr1 = (orc.OrcState * 1)(thetas[0])
-        z = (orc.I64 * 4)()
         t0 = time.perf_counter()
-        R.ref_sampled_sweep(arr1, 1, t, Larr, k, 32, 1, 2, z)
+        orc.sampled_sweep(thetas[:1], t, L, k, nphi=2, threads=1, lib_ref=lib)
         per_ip = (time.perf_counter() - t0) / 2
         nphi = int(max(2, min(1 << k, budget_s / per_ip)))
-        arr = (orc.OrcState * threads)(*thetas)
-        z = (orc.I64 * (4 * threads))()
         t0 = time.perf_counter()
-        n = R.ref_sampled_sweep(arr, threads, t, Larr, k, 32, threads, nphi, z)
+        cs, tm, n = orc.sampled_sweep(thetas, t, L, k, nphi=nphi, threads=threads, terms=True, lib_ref=lib)
         dt = time.perf_counter() - t0
+        if detail is not None:
+            detail.update({"nphi": nphi, "ref_csum": cs, "ref_terms": tm, "kind": "reference"})
         return {"value": n / dt, "unit": "inner products/s", "cores": threads, "kind": "reference",
                 "sample": f"{threads} projected thetas x first {nphi} of {1 << k} |L> states at t={t} "
                           f"({n} inner products, reference stabilizer.c/stateprep.c built "
@@ -101,6 +126,96 @@ def cpu_baseline(P_tuple, L, t, k, seed, budget_s=15.0, lib="libref.so", threads
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "inner products/s", "cores": 1, "kind": "port",
             "sample": f"{n} inner products theta' x prepL at t={t} by the oracle, 1 thread, {dt:.1f} s"}
+
+
+def zsum_partial(terms, nphi):
+    """Z[w] sums (scale 2^32) of the first nphi packed terms of each row, as
+    the engine and the oracle accumulate them"""
+    import numpy as np
+    out = np.zeros((terms.shape[0], 4), dtype=np.int64)
+    for r in range(terms.shape[0]):
+        c = [0, 0, 0, 0]
+        for v in terms[r, :nphi].tolist():
+            if not v:
+                continue
+            v -= 1
+            p, m = -(v >> 3), v & 7
+            e = 32 + (p >> 1)
+            val = 1 << e
+            parts = [(m, 1)] if not (p & 1) else [((m + 1) & 7, 1), ((m + 3) & 7, -1)]
+            for q, sg in parts:
+                if q >= 4:
+                    q, sg = q - 4, -sg
+                c[q] += sg * val
+        out[r] = c
+    return out
+
+
+def subsample_estimator(t, zsums, sexps):
+    """mean over the given samples of 2^t |pf sum_i <phi_i|theta'>|^2 from exact
+    Z[w] sums (pf = 2^{-sexp/2}), the exact value rounded once"""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 80
+    r2 = Decimal(2).sqrt()
+    tot = Decimal(0)
+    for s, c in zip(sexps, zsums):
+        if s < 0:
+            continue
+        c0, c1, c2, c3 = (int(v) for v in c)
+        A = c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3
+        B = c0 * c1 - c0 * c3 + c1 * c2 + c2 * c3
+        tot += (Decimal(A) + Decimal(B) * r2) * (Decimal(2) ** (t - s - 64))
+    return float(tot / len(sexps))
+
+
+def parity_leg(t, k, L, detail, gpu, threads):
+    """Untimed check of the engine against the reference C and the oracle on
+    the cpu_baseline's thetas (the first non-vanishing draws of timed step
+    0): the reference's per-term (eps, p, m mod 8) and partial sums over
+    j < nphi, the oracle's full per-theta sums over all chi terms, the
+    projection exponents, and the estimator over these thetas from the
+    engine's int64 and Kahan-FP32 paths against the oracle's FP64 value."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    idx = detail["index"]
+    chi = 1 << k
+    g_terms = gpu["terms"][idx]
+    g_sexp = gpu["sexp"]
+    g_zsum = gpu["zsum"][idx]
+    g_kahan = gpu["kahan"][idx]
+    out = {"thetas": len(idx), "theta_indices": idx}
+    # every draw before the last checked one: the engine's projection
+    # vanishes exactly where the oracle's does, with the same exponent
+    want_sexp = np.full(len(g_sexp), -1, dtype=np.int32)
+    want_sexp[idx] = detail["sexp"]
+    out["projection_exponent_mismatches"] = int(np.count_nonzero(g_sexp != want_sexp))
+    out["vanished_terms_nonzero"] = int(np.count_nonzero(gpu["terms"][want_sexp < 0]))
+    if detail.get("kind") == "reference":
+        nphi = detail["nphi"]
+        out["reference"] = "reference stabilizer.c/stateprep.c compiled from source (oracle/_ref)"
+        out["reference_terms_checked"] = int(len(idx) * nphi)
+        out["reference_term_mismatches"] = int(np.count_nonzero(g_terms[:, :nphi] != detail["ref_terms"]))
+        out["reference_partial_sum_mismatches"] = int(
+            np.count_nonzero((zsum_partial(g_terms, nphi) != detail["ref_csum"]).any(axis=1)))
+    t0 = time.perf_counter()
+    cs, tm, n = orc.sampled_sweep(detail["thetas"], t, L, k, nphi=chi, threads=threads, terms=True)
+    out["oracle_terms_checked"] = int(n)
+    out["oracle_term_mismatches"] = int(np.count_nonzero(g_terms != tm))
+    out["oracle_sum_mismatches"] = int(np.count_nonzero((g_zsum != cs).any(axis=1)))
+    out["oracle_s"] = round(time.perf_counter() - t0, 2)
+    se = detail["sexp"]
+    fp64 = subsample_estimator(t, cs, se)
+    i64 = subsample_estimator(t, g_zsum, g_sexp[idx])
+    kv = [2.0 ** (t - s) * (kr * kr + ki * ki) for s, (kr, ki) in zip(g_sexp[idx].tolist(), g_kahan.tolist())]
+    kah = sum(kv) / len(kv)
+    out["estimator"] = {"fp64_oracle": fp64, "int64_exact": i64, "kahan_fp32": kah,
+                        "abs_err_int64": abs(i64 - fp64), "abs_err_kahan_fp32": abs(kah - fp64)}
+    bad = [key for key in ("projection_exponent_mismatches", "vanished_terms_nonzero", "reference_term_mismatches",
+                           "reference_partial_sum_mismatches", "oracle_term_mismatches", "oracle_sum_mismatches")
+           if out.get(key, 0)]
+    out["mismatches"] = int(sum(out.get(key, 0) for key in bad))
+    return out
 
 
 def chunk_arg(args, default):
@@ -379,6 +494,12 @@ def main():
                                                        threads=args.cpu_threads)}), flush=True)
         return
     chi = 1 << k
+    check = not args.no_cpu_baseline
+    cpu_threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+    if check:
+        # the thetas of the CPU baseline and the parity leg: the first
+        # non-vanishing draws of timed step 0 (seed 1004)
+        _, _, _, ncheck = baseline_thetas(P_tuple, t, 1004, cpu_threads)
     with Engine(0) as E:
         cus, devname = E.device_info()
         chunk = chunk_arg(args, 0)
@@ -412,10 +533,13 @@ def main():
         value = ips / dt
         kernel_ips = ips / (kernel_ms / 1000.0)
 
-        # precision study on the last step's thetas
-        oracle = exact_mean(t, ex["sexp"], ex["zsum"])
-        err_int64 = abs(res.mean - oracle)
-        err_kahan = abs(res.mean_kahan - oracle)
+        # the exact value of the last step's estimator from its exact
+        # per-sample sums, rounded once: the int64 path's conversion and the
+        # Kahan-FP32 path against it (the sums themselves are checked against
+        # the reference and the oracle in the parity leg)
+        exact_last = exact_mean(t, ex["sexp"], ex["zsum"])
+        err_int64 = abs(res.mean - exact_last)
+        err_kahan = abs(res.mean_kahan - exact_last)
 
         # plain: the general per-pair kernel, one launch per pair
         npairs = args.plain_pairs
@@ -428,6 +552,11 @@ def main():
         _, plain_ms = E.inner_products(t, A, B, one_launch_per_pair=True)
         _, general_batched_ms = E.inner_products(t, A, B, one_launch_per_pair=False)
         plain_ips = npairs / (plain_ms / 1000.0)
+
+        # untimed: every term of timed step 0's first draws, for the parity leg
+        gpu_check = None
+        if check:
+            _, gpu_check = E.sampled_projector(P, L, k, 0, 1004, 0, ncheck, chunk=chunk, want_terms=True)
 
     out = {
         "metric": "inner products/s, batched vs plain, t=48 on 1 GPU; |err| vs FP64 oracle",
@@ -453,29 +582,44 @@ def main():
         "plain_pairs": npairs,
         "general_batched_ip_per_s": npairs / (general_batched_ms / 1000.0),
         "batched_vs_plain": value / plain_ips,
-        "estimator": {"int64_exact": res.mean, "kahan_fp32": res.mean_kahan, "fp64_oracle": oracle,
+        "estimator": {"int64_exact": res.mean, "kahan_fp32": res.mean_kahan,
+                      "exact_value_rounded_once": exact_last,
                       "abs_err_int64": err_int64, "abs_err_kahan_fp32": err_kahan},
     }
     if "tune_s" in tuning:
         out["tuning"] = tuning
-    # integer-VALU roofline of the sweep kernel (int32 lane-operations/s)
-    achieved = kernel_ips * VALU_LANE_OPS_PER_IP / 1e12
+    # integer-VALU roofline of the sweep kernel: the algorithm's own work
+    # (int32 lane-operations per inner product of the elimination algorithm
+    # it runs) at the kernel's rate, against the VALU peak
+    achieved = kernel_ips * ALG_LANE_OPS_PER_IP / 1e12
     out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_TOPS, "unit": "Tops/s",
                        "frac": achieved / PEAK_INT_TOPS,
                        "traffic": SWEEP_BYTES_PER_IP * M * chi,
+                       "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of one sweep dispatch, calibrated",
                        "kernel": "k_sweep_packed",
-                       "per_unit": f"{VALU_LANE_OPS_PER_IP:.0f} VALU lane-ops per inner product (measured)"}
+                       "per_unit": f"{ALG_LANE_OPS_PER_IP:.0f} int32 lane-ops ({ALG_WORDOPS_PER_IP:.0f} 64-bit "
+                                   f"word-ops) per inner product of the fixation/elimination algorithm",
+                       "canonical_wordops_per_ip": CANONICAL_WORDOPS_PER_IP}
     # the binding limit: instruction issue (one SALU per CU per cycle, one
-    # wave64 VALU per SIMD per 4 cycles)
+    # wave64 VALU per SIMD per 4 cycles), from the executed instruction counts
     out["issue"] = {
         "salu_frac": kernel_ips * SALU_INSTS_PER_IP / (cus * CLOCK_GHZ * 1e9),
         "valu_frac": kernel_ips * VALU_INSTS_PER_IP / (cus * 4 * CLOCK_GHZ * 1e9 / 4),
         "salu_per_ip": SALU_INSTS_PER_IP, "valu_per_ip": VALU_INSTS_PER_IP,
+        "executed_valu_lane_ops_per_ip": VALU_LANE_OPS_PER_IP,
         "hbm_GBps": kernel_ips * SWEEP_BYTES_PER_IP / 1e9,
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(P_tuple, L, t, k, 1004, lib=reflib, threads=args.cpu_threads)
+    if check:
+        detail = {}
+        out["cpu_baseline"] = cpu_baseline(P_tuple, L, t, k, 1004, lib=reflib, threads=cpu_threads, detail=detail)
+        par = parity_leg(t, k, L, detail, gpu_check, cpu_threads)
+        # the estimator over the checked thetas against the oracle's FP64 value
+        out["estimator"]["vs_fp64_oracle"] = dict(par.pop("estimator"), samples=len(detail["index"]))
+        out["parity"] = par
     print(json.dumps(out), flush=True)
+    if check and out["parity"]["mismatches"]:
+        print(f"bench.py: parity leg found {out['parity']['mismatches']} mismatches", file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

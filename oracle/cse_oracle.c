@@ -10,6 +10,7 @@
 #include "cse_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -654,4 +655,60 @@ long orc_ip_wordops(const orc_state *s1, const orc_state *s2)
     ops += 6L * st.k * s2->k;
     ops += 8L * st.k * st.k;
     return ops;
+}
+
+/* ------------------------------------------------------------------ */
+/* Sampled-mode sweep for given projected thetas (innerprod.c:119-139):
+ * per theta, sum_{i < nphi} <prepL(i) | theta'> as Z[w] fixed-point sums at
+ * scale 2^E, and optionally every term packed as 0 (eps = 0) or
+ * (-p) * 8 + (m mod 8) + 1 at terms[theta * nphi + i].  nthreads POSIX
+ * threads split the thetas cyclically (the rank-strided loop of
+ * probability.c:268).  Returns the number of inner products. */
+typedef struct {
+    const orc_state *thetas;
+    int ntheta, t, k, E, nphi, nthreads, tid;
+    const uint64_t *L;
+    int64_t *csum;
+    int32_t *terms;
+    long count;
+} orc_sweep_job;
+
+static void *orc_sweep_worker(void *arg)
+{
+    orc_sweep_job *j = (orc_sweep_job *)arg;
+    for (int th = j->tid; th < j->ntheta; th += j->nthreads) {
+        int64_t c[4] = {0, 0, 0, 0};
+        for (int i = 0; i < j->nphi; i++) {
+            orc_state phi;
+            orc_prepL(&phi, (uint64_t)i, j->t, j->L, j->k);
+            int eps, p, m;
+            orc_innerProduct(&j->thetas[th], &phi, &eps, &p, &m);
+            orc_zomega_add(c, j->E, eps, p, m);
+            if (j->terms)
+                j->terms[(long)th * j->nphi + i] = eps ? (-p) * 8 + (((m % 8) + 8) % 8) + 1 : 0;
+            j->count++;
+        }
+        memcpy(&j->csum[4 * th], c, sizeof(c));
+    }
+    return NULL;
+}
+
+long orc_sampled_sweep(const orc_state *thetas, int ntheta, int t, const uint64_t *L, int k, int E,
+                       int nthreads, int nphi, int64_t *csum, int32_t *terms)
+{
+    if (nphi > (1 << k) || nphi < 0) nphi = 1 << k;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    orc_sweep_job jobs[256];
+    for (int i = 0; i < nthreads; i++) {
+        jobs[i] = (orc_sweep_job){thetas, ntheta, t, k, E, nphi, nthreads, i, L, csum, terms, 0};
+        pthread_create(&th[i], NULL, orc_sweep_worker, &jobs[i]);
+    }
+    long total = 0;
+    for (int i = 0; i < nthreads; i++) {
+        pthread_join(th[i], NULL);
+        total += jobs[i].count;
+    }
+    return total;
 }

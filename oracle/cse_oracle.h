@@ -122,6 +122,13 @@ void orc_zomega_norm2(const int64_t c[4], __int128 *A, __int128 *B);
  * dense-row cost model documented in the engine's design notes. */
 long orc_ip_wordops(const orc_state *s1, const orc_state *s2);
 
+/* ---- sampled-mode sweep over given projected thetas (innerprod.c:119-139) ----
+ * csum[4 * theta] = sum_{i < nphi} <prepL(i) | theta'> at Z[w] scale 2^E;
+ * terms (may be NULL): packed per term as in the engine's per-term output.
+ * nthreads POSIX threads; returns the number of inner products. */
+long orc_sampled_sweep(const orc_state *thetas, int ntheta, int t, const uint64_t *L, int k, int E,
+                       int nthreads, int nphi, int64_t *csum, int32_t *terms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -133,13 +133,15 @@ void ref_prepL(orc_state *s, int i, int t, const uint64_t *L, int k)
  * nthreads POSIX threads splitting the theta range cyclically like the
  * reference's rank-strided loop (probability.c:268).  The projected thetas
  * are supplied by the caller in the oracle layout.  Returns the number of
- * inner products computed; per-theta exact sums go to csum[4*theta].
+ * inner products computed; per-theta exact sums go to csum[4*theta], and
+ * (ref_sampled_sweep_terms) every term to terms[theta * nphi + i].
  */
 typedef struct {
     const orc_state *thetas;
     int ntheta, t, k, nthreads, tid, E, nphi;
     const uint64_t *L;
     int64_t *csum;
+    int32_t *terms;
     long count;
 } ref_job;
 
@@ -157,6 +159,8 @@ static void *ref_worker(void *arg)
             int eps, p, m;
             innerProductExact(a, phi, &eps, &p, &m);
             ref_zomega_add(c, j->E, eps, p, m);
+            if (j->terms)
+                j->terms[(long)th * j->nphi + i] = eps ? (-p) * 8 + (((m % 8) + 8) % 8) + 1 : 0;
             freeStabilizerState(phi);
             j->count++;
         }
@@ -185,8 +189,10 @@ static void ref_zomega_add(int64_t c[4], int E, int eps, int p, int m)
     }
 }
 
-long ref_sampled_sweep(const orc_state *thetas, int ntheta, int t, const uint64_t *L, int k,
-                       int E, int nthreads, int nphi, int64_t *csum)
+/* as ref_sampled_sweep, and every term packed as 0 (eps = 0) or
+ * (-p) * 8 + (m mod 8) + 1 at terms[theta * nphi + i] when terms != NULL */
+long ref_sampled_sweep_terms(const orc_state *thetas, int ntheta, int t, const uint64_t *L, int k,
+                             int E, int nthreads, int nphi, int64_t *csum, int32_t *terms)
 {
     if (nphi > (1 << k) || nphi < 0) nphi = 1 << k;
     if (nthreads < 1) nthreads = 1;
@@ -194,7 +200,7 @@ long ref_sampled_sweep(const orc_state *thetas, int ntheta, int t, const uint64_
     ref_job jobs[256];
     if (nthreads > 256) nthreads = 256;
     for (int i = 0; i < nthreads; i++) {
-        jobs[i] = (ref_job){thetas, ntheta, t, k, nthreads, i, E, nphi, L, csum, 0};
+        jobs[i] = (ref_job){thetas, ntheta, t, k, nthreads, i, E, nphi, L, csum, terms, 0};
         pthread_create(&th[i], NULL, ref_worker, &jobs[i]);
     }
     long total = 0;
@@ -203,4 +209,10 @@ long ref_sampled_sweep(const orc_state *thetas, int ntheta, int t, const uint64_
         total += jobs[i].count;
     }
     return total;
+}
+
+long ref_sampled_sweep(const orc_state *thetas, int ntheta, int t, const uint64_t *L, int k,
+                       int E, int nthreads, int nphi, int64_t *csum)
+{
+    return ref_sampled_sweep_terms(thetas, ntheta, t, L, k, E, nthreads, nphi, csum, NULL);
 }

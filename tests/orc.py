@@ -101,6 +101,9 @@ def lib():
         L.orc_zomega_add.argtypes = [P(I64), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_ip_wordops.argtypes = [P(OrcState), P(OrcState)]
         L.orc_ip_wordops.restype = ctypes.c_long
+        L.orc_sampled_sweep.argtypes = [P(OrcState), ctypes.c_int, ctypes.c_int, P(U64), ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, P(I64), P(ctypes.c_int32)]
+        L.orc_sampled_sweep.restype = ctypes.c_long
         _lib = L
     return _lib
 
@@ -126,6 +129,10 @@ def ref(name="libref.so"):
         R.ref_sampled_sweep.argtypes = [P(OrcState), ctypes.c_int, ctypes.c_int, P(U64), ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, P(I64)]
         R.ref_sampled_sweep.restype = ctypes.c_long
+        R.ref_sampled_sweep_terms.argtypes = [P(OrcState), ctypes.c_int, ctypes.c_int, P(U64), ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_int, P(I64),
+                                              P(ctypes.c_int32)]
+        R.ref_sampled_sweep_terms.restype = ctypes.c_long
         _ref[name] = R
     return _ref[name]
 
@@ -311,3 +318,24 @@ def parse_list(line):
 def load_json_fixture(name):
     with gzip.open(os.path.join(GOLDEN, "ref", name), "rt") as f:
         return json.load(f)
+
+
+def sampled_sweep(thetas, t, L, k, nphi=None, threads=1, terms=False, lib_ref=None):
+    """Per-theta Z[w] sums (scale 2^32) of sum_{i < nphi} <prepL(i)|theta'> for
+    projected thetas, by the oracle (lib_ref None) or by the reference C
+    (lib_ref = a libref name); with terms, also every packed term.
+    Returns (csum as an (ntheta, 4) int64 array, terms as (ntheta, nphi)
+    int32 or None, inner products computed)."""
+    nphi = (1 << k) if nphi is None else int(nphi)
+    n = len(thetas)
+    arr = (OrcState * n)(*thetas)
+    Larr = (U64 * max(1, k))(*L)
+    cs = np.zeros((n, 4), dtype=np.int64)
+    tm = np.zeros((n, nphi), dtype=np.int32) if terms else None
+    tp = tm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if terms else None
+    cp = cs.ctypes.data_as(ctypes.POINTER(I64))
+    if lib_ref is None:
+        cnt = lib().orc_sampled_sweep(arr, n, t, Larr, k, 32, threads, nphi, cp, tp)
+    else:
+        cnt = ref(lib_ref).ref_sampled_sweep_terms(arr, n, t, Larr, k, 32, threads, nphi, cp, tp)
+    return cs, tm, cnt

@@ -117,6 +117,24 @@ def to_record(th):
 END = 60  # the end row of the elimination loop
 TOP = 63  # kept in act: bit 63 of eta is D_a's top bit
 U64 = (1 << 64) - 1
+LIVE = (1 << END) - 1  # the variables' bits (the end row and bit 63 excluded)
+
+# Algorithmic work of the fixation/elimination algorithm, in 64-bit word
+# operations of a dense-row formulation (one operation per live row and per
+# row-level operation of a step), counted under count["w_*"]:
+#   fixation of the pivot coordinates outside supp(x_j): per record row, the
+#     AND with the fixed set, its parity and the D update: 3 per row;
+#   restriction on a non-pivot column: the column's bits over the live rows
+#     (2 per row), then, unless it is empty, a constrained elimination (5 per
+#     live row, as a B step);
+#   A step (D_a in {2,6}): the pivot, then per live row the membership in
+#     eta, J ^= eta and D -= 2: 1 + 3 per live row;
+#   B step (D_a in {0,4} with a constraint): the pivot, then per live row the
+#     membership in Sp, J ^= J_i, the D update, the membership in M and
+#     J ^= Sp: 1 + 5 per live row;
+#   C step (no constraint) and the loop's end: 1;
+#   the batch of uncoupled variables (exact mode): one pass, 2 per live row.
+W_STEP = {"A": 3, "B": 5}
 
 
 def iso_batch(J, act, W, count):
@@ -148,12 +166,14 @@ def elim_loop(J, act, W, count, iso=False):
     first C step."""
     while True:
         a = low(act)
+        live = popc(act & LIVE)
         act &= ~(1 << a)
         Ja = J[a]
         hi = Ja >> 32
         eta = Ja & act                           # bit 63: D_a >> 2
         if (hi >> 30) & 1:                       # D_a in {2,6}
             count["A"] += 1
+            count["w_A"] += 1 + W_STEP["A"] * live
             for x in bits(eta):
                 lo = (J[x] & U32) ^ (eta & U32)
                 h2 = (((J[x] >> 32) ^ (eta >> 32)) + 0xC0000000) & U32
@@ -164,16 +184,20 @@ def elim_loop(J, act, W, count, iso=False):
             if (hi >> 31) & 1:                   # D_a = 4, or the end row
                 if a != END:
                     count["vanish"] += 1
+                count["w_C"] += 1
                 return a, W
             count["C"] += 1
+            count["w_C"] += 1
             W = (W + 0xFFFFFFF0) & U32
             if iso:
                 iso = False
+                count["w_iso"] += 2 * popc(act & LIVE)
                 act, W, gone = iso_batch(J, act, W, count)
                 if gone:
                     return 0, W
             continue
         count["B"] += 1                          # constraint eta . z = D_a/4
+        count["w_B"] += 1 + W_STEP["B"] * live
         act &= ~(1 << low(eta))
         i = low(eta)
         Jr = J[i]
@@ -283,6 +307,7 @@ def batch_ip(C, xj, count):
     J0 = C["J0"]
     B = C["actH"] & ~xj
     act = C["actR"] & xj
+    count["w_fix"] += 3 * popc(C["actR"] & LIVE)
     J, E = [0] * 64, 0
     for x in range(64):
         jb = J0[x] & B
@@ -299,6 +324,7 @@ def batch_ip(C, xj, count):
         while True:
             count["restrict"] += 1
             S = col & act
+            count["w_res"] += (2 + (5 if S else 0)) * popc(act & LIVE)
             beta = (popc(col & B) ^ h) & 1
             if S:
                 i = low(S)
@@ -330,6 +356,7 @@ def batch_ip_gen(C, r, xj, count):
     J0 = C["J0"]
     B = C["actH"] & ~xj
     act = C["actR"] & xj
+    count["w_fix"] += 3 * popc(C["actR"] & LIVE)
     J, E = [0] * 64, 0
     for x in range(64):
         jb = J0[x] & B
@@ -346,6 +373,7 @@ def batch_ip_gen(C, r, xj, count):
             count["rg"] += 1
             col = sum(1 << x for x in range(64) if (G[x] >> q) & 1)
             S = col & act
+            count["w_res"] += (2 + (5 if S else 0)) * popc(act & LIVE)
             c = (popc(col & B) ^ (h >> q)) & 1
             if not S:
                 if c:
@@ -392,6 +420,7 @@ def fallback_ip(r, xj, n, count):
     F = r.act & ~xj
     B = F & r.h
     s = r.copy()
+    count["w_fix"] += 3 * popc(r.act)
     edges = sum(1 for a in bits(B) for b in bits(B) if a < b and (r.J[a] >> b) & 1)
     s.Q = (r.Q + sum(r.d[a] for a in bits(B)) + 4 * edges) & 7
     for x in range(64):
@@ -403,6 +432,7 @@ def fallback_ip(r, xj, n, count):
         count["restrict"] += 1
         col = sum(1 << x for x in range(64) if (s.G[x] >> q) & 1)
         S = col & s.act
+        count["w_res"] += (2 + (5 if S else 0)) * popc(s.act)
         beta = ((s.h >> q) ^ popc(col & B)) & 1
         if not S:
             if beta:
@@ -890,3 +920,51 @@ def test_wave_instance_search_matches_bisect(n):
     # two dependent loads up to 4096 instances, three up to 262144
     assert worst <= (0 if n == 1 else 1 if n <= 64 else 2 if n <= 4096 else 3)
 
+
+
+def algorithmic_work(t, r, k, seed, nth, nj):
+    """Mean word operations per inner product of the fixation/elimination
+    algorithm (the cost model above) on the sampled-mode workload of
+    sampled_config(t, r, k, seed): the first nth non-vanishing projected
+    random states of the seed's stream, each against nj seeded j, through
+    the path the sweep takes (phiL_batch for records with at most two
+    non-pivot columns, else the fallback).  Returns (mean, per-kind means,
+    per-IP event counts)."""
+    from circuitsimulator_amd.workloads import sampled_config
+    _, (ph, xs, zs), L = sampled_config(t, r, k, seed=seed)
+    Po = orc.make_projector(ph, xs, zs, t)
+    thr = orc.thresholds(t)
+    rng = np.random.default_rng(seed + 1)
+    js = [int(v) for v in rng.integers(0, 1 << k, size=nj)]
+    count = collections.Counter()
+    ips, st, done = 0, 0, 0
+    while done < nth:
+        th = orc.random_state(t, seed, st, thr)
+        st += 1
+        if orc.project(th, Po) < 0:
+            continue
+        done += 1
+        rec = to_record(th)
+        C = batch_consts(rec, t)
+        batch = popc(((1 << t) - 1) & ~rec.act) <= 2
+        for j in js:
+            xj = xj_of(j, L, k)
+            if batch:
+                batch_ip(C, xj, count)
+            else:
+                fallback_ip(rec, xj, t, count)
+            ips += 1
+    kinds = {key: v / ips for key, v in count.items() if key.startswith("w_")}
+    events = {key: v / ips for key, v in count.items() if not key.startswith("w_")}
+    return sum(kinds.values()), kinds, events
+
+
+def test_algorithmic_work_of_the_headline_workload():
+    # the per-IP figure the benchmark's roofline uses: the t=48, r=24, k=12
+    # norm estimation (seed 1004), 64 projected random states x 512 seeded j
+    import bench
+    mean, kinds, events = algorithmic_work(48, 24, 12, 1004, 64, 512)
+    assert round(mean, 1) == bench.ALG_WORDOPS_PER_IP
+    # the step statistics behind it: about 7.6 A and 7.1 B steps per IP
+    assert 7.0 < events["A"] < 8.3 and 6.5 < events["B"] < 7.8
+    assert kinds["w_fix"] == pytest.approx(141.5, abs=0.5)
