@@ -41,7 +41,9 @@ def projector_payload(projector):
 
 def make_payload(G, H, config=None, t=None):
     """stdin payload (probability.py:247-272).  config keys as in the
-    reference's probability() config dict."""
+    reference's probability() config dict; unset values are sent as 0, as
+    the reference's send() does (samples and bins included: probability()
+    fills them in only when sampling is allowed)."""
     c = dict(config or {})
     if t is None:
         t = len(G[1][0]) if len(G[0]) else (len(H[1][0]) if len(H[0]) else 0)
@@ -49,8 +51,8 @@ def make_payload(G, H, config=None, t=None):
     s += _line(1 if c.get("quiet") else 0)
     s += _line(1 if c.get("verbose") else 0)
     s += _line(1 if c.get("noapprox") else 0)
-    s += _line(c.get("samples", 2000))
-    s += _line(c.get("bins", 1))
+    s += _line(c.get("samples"))
+    s += _line(c.get("bins"))
     s += _line(t)
     s += _line(c.get("k"))
     s += _line(1 if c.get("exact", True) else 0)
