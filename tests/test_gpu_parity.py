@@ -263,6 +263,59 @@ def test_uneven_workload_seeded_subsample(E):
     assert all(r.im == 0.0 for r in res)
 
 
+def test_uneven_workload_large_k_subsample(E):
+    # a seeded 3% subsample of the uneven batch workload with no filter on k:
+    # instances taken in a seeded order until 40 of them have k >= 11 (those
+    # hold 99.6% of the workload's terms), 123 in all, k = 4..14.  They run in
+    # launches of at most 2^28 pair terms with every term written out, and
+    # per instance 100 seeded terms (always the first and last) are checked
+    # against the oracle.
+    from concurrent.futures import ThreadPoolExecutor
+
+    from circuitsimulator_amd.workloads import uneven_instances
+    inst, specs = uneven_instances(4096, seed=1003)
+    rng = np.random.default_rng(3004)
+    pick, big = [], 0
+    for q in rng.permutation(len(inst)):
+        pick.append(int(q))
+        big += specs[q][2] >= 11
+        if big == 40:
+            break
+    assert len(pick) == 123 and max(specs[q][2] for q in pick) == 14
+    size = {q: (1 << specs[q][2]) * ((1 << specs[q][2]) + 1) // 2 for q in pick}
+    groups, cur = [], []
+    for q in sorted(pick, key=lambda q: size[q]):
+        if cur and sum(size[c] for c in cur) + size[q] > (1 << 28):
+            groups.append(cur)
+            cur = []
+        cur.append(q)
+    groups.append(cur)
+    checks = []
+    for g in groups:
+        print(f"large-k subsample: launch of {len(g)} instances, k = {sorted({specs[q][2] for q in g})}", flush=True)
+        res, terms, _ = E.batch_exact([inst[q] for q in g], want_terms=True, nterms=sum(size[q] for q in g))
+        assert all(r.im == 0.0 and r.re >= 0.0 for r in res)
+        assert [r.terms for r in res] == [size[q] for q in g]
+        off = 0
+        for q in g:
+            nl = size[q]
+            ls = sorted(set([0, nl - 1] + [int(v) for v in rng.integers(0, nl, size=98)]))
+            checks.append((q, ls, [int(terms[off + l]) for l in ls]))
+            off += nl
+        del terms
+
+    def oracle(c):
+        q, ls, got = c
+        t, r, k, ph, xs, zs, L = specs[q]
+        want = [w[0] for w in oracle_exact_terms(t, k, [int(v) for v in L], 0, ph, xs, zs, ls=ls)]
+        return q, got == want
+
+    with ThreadPoolExecutor(8) as pool:
+        bad = [q for q, ok in pool.map(oracle, checks) if not ok]
+    assert not bad, bad
+    assert sum(len(c[1]) for c in checks if specs[c[0]][2] >= 11) >= 40 * 99
+
+
 def test_sampled_t64_unpacked_layout(E):
     # n > 61 runs the sweep with D in its own register
     from circuitsimulator_amd.engine import Projector
