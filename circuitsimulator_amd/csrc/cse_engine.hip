@@ -16,6 +16,8 @@
 #include "../../include/cse.h"
 #include "cse_host.h"
 
+static_assert(HIST_STRIDE == CSE_HIST_BINS, "device histogram stride is the public histogram size");
+
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
@@ -59,11 +61,15 @@ struct cse_ctx {
 
 namespace {
 
+// The message is formatted into a local buffer first: an argument may be
+// the context's own previous message (fail(ctx, rc, "...: %s", ctx->err)).
 int fail(cse_ctx *c, int code, const char *fmt, ...) {
+    char msg[sizeof(c->err)];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    vsnprintf(msg, sizeof(msg), fmt, ap);
     va_end(ap);
+    memcpy(c->err, msg, sizeof(msg));
     return code;
 }
 
@@ -766,9 +772,9 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
         for (int i = lo; i < hi; i++) {
             cse_exact_result *r = &res[i];
             const int64_t *h = (const int64_t *)&hist[(size_t)HIST_STRIDE * i];
-            r->pmin = -128;
+            r->pmin = CSE_HIST_PMIN;
             if (r->hist) memcpy(r->hist, h, 8 * (size_t)HIST_STRIDE);
-            cse_hist_value(h, -128, &r->re, &r->im);
+            cse_hist_value(h, CSE_HIST_PMIN, &r->re, &r->im);
             r->value = hypot(r->re, r->im);
             r->kahan_re = kz[2 * i];
             r->kahan_im = kz[2 * i + 1];

@@ -38,7 +38,8 @@ inline __host__ __device__ long long tri_items(long long chi, long long ch, long
     return tri_before(ch, nch, nch - 1) + (chi - (nch - 1) * ch);
 }
 
-// Histogram bins per instance: P in [-(64 + 64), 0], 8 phases.
+// Histogram bins per instance: P in [-(64 + 64), 0], 8 phases (the public
+// CSE_HIST_BINS).
 #define HIST_STRIDE (129 * 8)
 // Per-wave LDS histogram: p in [-64, 0], 8 phases.
 #define LH_BINS (65 * 8)

@@ -89,12 +89,17 @@ int cse_random_states(cse_ctx *ctx, int n, uint64_t seed, int64_t first, size_t 
                       const cse_projector *P, uint64_t *out, int32_t *sexp);
 
 /* ---------------------------------------------------------------- results */
+/* Exact-mode histograms: P = p - s runs over [CSE_HIST_PMIN, 0] (p >= -64 per
+ * inner product, s <= 64 projection factors), 8 phases each. */
+#define CSE_HIST_PMIN (-128)
+#define CSE_HIST_BINS ((1 - CSE_HIST_PMIN) * 8)   /* 1032 */
+
 typedef struct {
     /* exact accumulation: counts of terms 2^{P/2} e^{i pi M/4} per bin
      * hist[(-P)*8 + M], P in [pmin, 0]; off-diagonal terms 2 Re(z) are
      * counted as z + conj(z) (innerprod.c:254-260) */
-    int pmin;
-    int64_t *hist;      /* caller buffer of (1 - pmin) * 8 entries, or NULL */
+    int pmin;           /* output: always CSE_HIST_PMIN */
+    int64_t *hist;      /* caller buffer of CSE_HIST_BINS entries, or NULL */
     double re, im;      /* the exact total rounded once to FP64 */
     double value;       /* |total| -- exactProjector's return (innerprod.c:198) */
     double kahan_re, kahan_im; /* Kahan-compensated FP32 sums, reduced in FP64 */
