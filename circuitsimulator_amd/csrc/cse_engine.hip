@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -55,9 +56,46 @@ struct cse_ctx {
     int *d_err = nullptr;
     HostBuf hstage;  // results coming back (per-sample sums, histograms)
     HostBuf hup;     // the plan going up
+    // the memory the current batched call works in: the buffers above, or
+    // the static scratch array (cse_exact_projector_noalloc)
+    char *b_plan = nullptr, *b_out = nullptr, *b_hup = nullptr, *b_hstage = nullptr;
+    uint64_t *b_recs = nullptr;
+    unsigned long long *b_queue = nullptr;
+    int32_t *b_sexp1 = nullptr, *b_terms = nullptr;
+    // this device's copy of the static scratch array (nullptr if its address
+    // could not be obtained; scratch_err says why)
+    char *scratch = nullptr;
+    char scratch_err[160] = {0};
     bool thr_ready = false;
     int blocks_per_cu = 8;  // persistent grid: blocks of 4 waves per CU
 };
+
+// ------------------------------------------------- static scratch array
+// cse_exact_projector_noalloc works in this file-scope device array instead
+// of allocated buffers: the work queue, the plan, the results block, the
+// projection exponents and the records of one instance of chi <= 2^13.  Each
+// device holds its own copy for the life of the process; its address is
+// taken once per device with hipGetSymbolAddress when the first context on
+// that device is created, and host code only ever uses that address.
+constexpr int kStaticMaxLogChi = 13;
+constexpr long long kStaticMaxChi = 1ll << kStaticMaxLogChi;
+constexpr size_t kStaticPlanBytes = 4096;  // one instance's plan: 2,816 B
+constexpr size_t kStaticOutBytes = 8448;   // one instance's results block: 8,288 B
+constexpr size_t kScratchQueue = 0;
+constexpr size_t kScratchPlan = 256;
+constexpr size_t kScratchOut = kScratchPlan + kStaticPlanBytes;
+constexpr size_t kScratchSexp = kScratchOut + kStaticOutBytes;
+constexpr size_t kScratchRecs = kScratchSexp + 4 * (size_t)kStaticMaxChi;
+constexpr size_t kScratchBytes = kScratchRecs + (size_t)REC_WORDS * 8 * (size_t)kStaticMaxChi;  // 8,761,856 B
+static_assert(kScratchOut % 256 == 0 && kScratchSexp % 256 == 0 && kScratchRecs % 256 == 0, "scratch parts aligned");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[kMaxScratchDevices];
+static bool g_scratch_tried[kMaxScratchDevices];
+static char g_scratch_msg[kMaxScratchDevices][128];
 
 namespace {
 
@@ -224,34 +262,73 @@ int sweep_grid(cse_ctx *ctx, long long nitems, int bpc) {
 // Records for instances [0, ninst): mode 0 random states (sample indices
 // first + i), mode 1 projected decomposition states.  Then the sweep.
 struct Plan {
-    std::vector<InstDesc> inst;
-    std::vector<long long> rec_off;   // ninst + 1
-    std::vector<long long> item_off;  // ninst + 1
-    std::vector<DevProjector> projs;
+    int ninst = 0;
+    InstDesc *inst = nullptr;
+    long long *rec_off = nullptr;   // ninst + 1
+    long long *item_off = nullptr;  // ninst + 1
+    DevProjector *projs = nullptr;
     long long nrec = 0, nitems = 0, nterms = 0;
+    // appends an instance of nr records, ni work items and nt terms (its
+    // record and term offsets are set here)
+    void add(InstDesc d, const DevProjector &p, long long nr, long long ni, long long nt) {
+        if (ninst == 0) rec_off[0] = item_off[0] = 0;
+        d.rec_off = nrec;
+        d.term_off = nterms;
+        inst[ninst] = d;
+        projs[ninst] = p;
+        nrec += nr;
+        nitems += ni;
+        nterms += nt;
+        ninst++;
+        rec_off[ninst] = nrec;
+        item_off[ninst] = nitems;
+    }
 };
 
-// The plan goes up through one pinned buffer in one DMA transfer, into one
-// device allocation of the same layout (256-byte aligned parts); every call
-// synchronizes its stream before returning, so the buffer is free again at
-// the next call.
-int upload_plan(cse_ctx *ctx, const Plan &pl) {
-    const int ninst = (int)pl.inst.size();
-    const size_t sz[4] = {sizeof(InstDesc) * ninst, sizeof(long long) * (ninst + 1),
-                          sizeof(long long) * (ninst + 1), sizeof(DevProjector) * ninst};
-    const void *src[4] = {pl.inst.data(), pl.item_off.data(), pl.rec_off.data(), pl.projs.data()};
-    size_t off[4], total = 0;
+// Heap storage of a plan of up to n instances.
+struct PlanStore {
+    std::vector<InstDesc> inst;
+    std::vector<long long> ro, io;
+    std::vector<DevProjector> pj;
+    // n = 0 allocates nothing (the static path brings its own storage)
+    explicit PlanStore(int n) : inst(n), ro(n ? n + 1 : 0), io(n ? n + 1 : 0), pj(n) {}
+    Plan plan() {
+        Plan p;
+        p.inst = inst.data();
+        p.rec_off = ro.data();
+        p.item_off = io.data();
+        p.projs = pj.data();
+        return p;
+    }
+};
+
+// Plan layout: instance descriptors, item and record offsets, projectors,
+// each part 256-byte aligned.  Returns the total size.
+size_t plan_layout(const Plan &pl, size_t off[4], size_t sz[4]) {
+    const size_t ninst = (size_t)pl.ninst;
+    sz[0] = sizeof(InstDesc) * ninst;
+    sz[1] = sizeof(long long) * (ninst + 1);
+    sz[2] = sizeof(long long) * (ninst + 1);
+    sz[3] = sizeof(DevProjector) * ninst;
+    size_t total = 0;
     for (int b = 0; b < 4; b++) {
         off[b] = total;
         total += (sz[b] + 255) & ~(size_t)255;
     }
-    ENSURE(plan, total);
-    int rc = ensure_host(ctx, ctx->hup, total);
-    if (rc) return rc;
-    char *h = (char *)ctx->hup.p;
+    return total;
+}
+
+// The plan goes up from the bound host stage in one transfer, into the bound
+// device plan of the same layout; every call synchronizes its stream before
+// returning, so the stage is free again at the next call.
+int upload_plan(cse_ctx *ctx, const Plan &pl) {
+    size_t off[4], sz[4];
+    const size_t total = plan_layout(pl, off, sz);
+    const void *src[4] = {pl.inst, pl.item_off, pl.rec_off, pl.projs};
+    char *h = ctx->b_hup;
     for (int b = 0; b < 4; b++) memcpy(h + off[b], src[b], sz[b]);
-    HIPCHK(hipMemcpyAsync(ctx->plan.p, h, total, hipMemcpyHostToDevice, ctx->stream));
-    char *d = (char *)ctx->plan.p;
+    HIPCHK(hipMemcpyAsync(ctx->b_plan, h, total, hipMemcpyHostToDevice, ctx->stream));
+    char *d = ctx->b_plan;
     ctx->d_inst = (InstDesc *)(d + off[0]);
     ctx->d_itemoff = (long long *)(d + off[1]);
     ctx->d_recoff = (long long *)(d + off[2]);
@@ -260,13 +337,13 @@ int upload_plan(cse_ctx *ctx, const Plan &pl) {
 }
 
 int run_records(cse_ctx *ctx, const Plan &pl, int mode) {
-    const int ninst = (int)pl.inst.size();
+    const int ninst = pl.ninst;
     const long long waves = pl.nrec;
     const long long blocks = (waves + 3) / 4;
     if (blocks > 0) {
         hipLaunchKernelGGL(k_records, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, mode, ninst,
                            ctx->d_recoff, ctx->d_inst, ctx->d_projs,
-                           dp<uint64_t>(ctx->thr), pl.nrec, dp<uint64_t>(ctx->recs), ctx->d_sexp,
+                           mode == 0 ? dp<uint64_t>(ctx->thr) : nullptr, pl.nrec, ctx->b_recs, ctx->d_sexp,
                            ctx->d_err);
         HIPCHK(hipGetLastError());
     }
@@ -283,7 +360,7 @@ struct OutLayout {
 };
 
 OutLayout out_layout(const Plan &pl, int mode) {
-    const size_t n = (size_t)pl.nrec, ni = pl.inst.size();
+    const size_t n = (size_t)pl.nrec, ni = (size_t)pl.ninst;
     OutLayout o;
     memset(&o, 0, sizeof(o));
     if (mode == 0) {
@@ -303,42 +380,85 @@ OutLayout out_layout(const Plan &pl, int mode) {
     return o;
 }
 
-// The batch's device buffers, and one launch that clears its results block
-// (every sum, count and flag; the exponents are rewritten by the record
-// launch) and its work queue.
-int prep_batch(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
+// Binds a batch to the context's own buffers, grown on demand (device memory
+// and pinned host stages).
+int bind_ctx_mem(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
+    size_t off[4], sz[4];
+    const size_t pbytes = plan_layout(pl, off, sz);
     const OutLayout o = out_layout(pl, mode);
+    ENSURE(plan, pbytes);
     ENSURE(outb, o.total);
     ENSURE(recs, (size_t)REC_WORDS * 8 * (size_t)pl.nrec);
     ENSURE(queue, 8);
     if (mode == 1) ENSURE(sexp, 4 * (size_t)pl.nrec);
     if (want_terms) ENSURE(terms, 4 * (size_t)(pl.nterms > 0 ? pl.nterms : 1));
-    char *b = (char *)ctx->outb.p;
+    int rc = ensure_host(ctx, ctx->hup, pbytes);
+    if (!rc) rc = ensure_host(ctx, ctx->hstage, o.total);
+    if (rc) return rc;
+    ctx->b_plan = (char *)ctx->plan.p;
+    ctx->b_out = (char *)ctx->outb.p;
+    ctx->b_recs = dp<uint64_t>(ctx->recs);
+    ctx->b_queue = dp<unsigned long long>(ctx->queue);
+    ctx->b_sexp1 = mode == 1 ? dp<int32_t>(ctx->sexp) : nullptr;
+    ctx->b_terms = want_terms ? dp<int32_t>(ctx->terms) : nullptr;
+    ctx->b_hup = (char *)ctx->hup.p;
+    ctx->b_hstage = (char *)ctx->hstage.p;
+    return CSE_OK;
+}
+
+// Binds a one-instance exact-mode batch to this device's static scratch
+// array and the caller's host stages; allocates nothing.
+int bind_static_mem(cse_ctx *ctx, const Plan &pl, char *hup, char *hstage) {
+    if (!ctx->scratch) return fail(ctx, CSE_ERR_HIP, "static scratch array unavailable: %s", ctx->scratch_err);
+    size_t off[4], sz[4];
+    const size_t pbytes = plan_layout(pl, off, sz);
+    const OutLayout o = out_layout(pl, 1);
+    if (pl.nrec > kStaticMaxChi)
+        return fail(ctx, CSE_ERR_ARG, "chi=%lld above the no-allocation limit 2^%d", pl.nrec, kStaticMaxLogChi);
+    if (pbytes > kStaticPlanBytes || o.total > kStaticOutBytes)
+        return fail(ctx, CSE_ERR_ARG, "internal: plan or results of %zu / %zu B above the static stages", pbytes,
+                    o.total);
+    ctx->b_queue = (unsigned long long *)(ctx->scratch + kScratchQueue);
+    ctx->b_plan = ctx->scratch + kScratchPlan;
+    ctx->b_out = ctx->scratch + kScratchOut;
+    ctx->b_sexp1 = (int32_t *)(ctx->scratch + kScratchSexp);
+    ctx->b_recs = (uint64_t *)(ctx->scratch + kScratchRecs);
+    ctx->b_terms = nullptr;
+    ctx->b_hup = hup;
+    ctx->b_hstage = hstage;
+    return CSE_OK;
+}
+
+// One launch that clears the bound results block (every sum, count and flag;
+// the exponents are rewritten by the record launch) and the work queue.
+int prep_batch(cse_ctx *ctx, const Plan &pl, int mode) {
+    const OutLayout o = out_layout(pl, mode);
+    char *b = ctx->b_out;
     ctx->out_bytes = o.total;
     ctx->d_zsum = mode == 0 ? (long long *)(b + o.zsum) : nullptr;
     ctx->d_kahan = (double *)(b + o.kahan);
-    ctx->d_sexp = mode == 0 ? (int32_t *)(b + o.sexp) : dp<int32_t>(ctx->sexp);
+    ctx->d_sexp = mode == 0 ? (int32_t *)(b + o.sexp) : ctx->b_sexp1;
     ctx->d_hist = mode == 1 ? (unsigned long long *)(b + o.hist) : nullptr;
     ctx->d_ips = (unsigned long long *)(b + o.ips);
     ctx->d_err = (int *)(b + o.err);
     ZeroList zl;
     memset(&zl, 0, sizeof(zl));
-    zl_add(zl, ctx->outb.p, (long long)(o.total / 8));
-    zl_add(zl, ctx->queue.p, 1);
+    zl_add(zl, b, (long long)(o.total / 8));
+    zl_add(zl, ctx->b_queue, 1);
     return zero_list(ctx, zl);
 }
 
 int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
-    const int ninst = (int)pl.inst.size();
+    const int ninst = pl.ninst;
     SweepArgs A;
     A.mode = mode;
     A.ninst = ninst;
     A.nitems = pl.nitems;
     A.item_off = ctx->d_itemoff;
     A.inst = ctx->d_inst;
-    A.recs = dp<uint64_t>(ctx->recs);
-    A.queue = dp<unsigned long long>(ctx->queue);
-    A.terms = want_terms ? dp<int32_t>(ctx->terms) : nullptr;
+    A.recs = ctx->b_recs;
+    A.queue = ctx->b_queue;
+    A.terms = want_terms ? ctx->b_terms : nullptr;
     A.zsum = ctx->d_zsum;
     A.kahan = ctx->d_kahan;
     A.hist = ctx->d_hist;
@@ -346,7 +466,7 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     A.ips = ctx->d_ips;
     A.err = ctx->d_err;
     bool packed = true;
-    for (const InstDesc &d : pl.inst) packed = packed && d.n <= CSE_END_ROW;
+    for (int i = 0; i < ninst; i++) packed = packed && pl.inst[i].n <= CSE_END_ROW;
     // exact mode takes k_sweep_packed_all (seven waves per SIMD); sampled
     // launches, |L> and |H^t> terms alike, take k_sweep_packed (eight)
     const bool all = packed && mode == 1;
@@ -415,6 +535,28 @@ int init_ctx(cse_ctx *ctx, int device) {
     snprintf(ctx->name, sizeof(ctx->name), "%s (%s)", prop.name, prop.gcnArchName);
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(ctx, CSE_ERR_HIP, "device %d is %s; this engine is built for gfx950", device, prop.gcnArchName);
+    // this device's copy of the static scratch array, once per device (with
+    // the device current); a failure only disables the no-allocation call
+    if (device < kMaxScratchDevices) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_tried[device]) {
+            g_scratch_tried[device] = true;
+            void *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e == hipSuccess && p) {
+                g_scratch_ptr[device] = (char *)p;
+            } else {
+                (void)hipGetLastError();
+                snprintf(g_scratch_msg[device], sizeof(g_scratch_msg[device]), "hipGetSymbolAddress: %s",
+                         e != hipSuccess ? hipGetErrorString(e) : "null address");
+            }
+        }
+        ctx->scratch = g_scratch_ptr[device];
+        if (!ctx->scratch) snprintf(ctx->scratch_err, sizeof(ctx->scratch_err), "%s", g_scratch_msg[device]);
+    } else {
+        snprintf(ctx->scratch_err, sizeof(ctx->scratch_err), "device index %d above %d", device,
+                 kMaxScratchDevices - 1);
+    }
     return CSE_OK;
 }
 
@@ -590,30 +732,22 @@ int cse_random_states(cse_ctx *ctx, int n, uint64_t seed, int64_t first, size_t 
 // of them, then each instance's estimator from its own samples.
 static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst, int chunk,
                          cse_sampled_result *res, double *kernel_ms) {
-    Plan pl;
-    pl.rec_off.push_back(0);
-    pl.item_off.push_back(0);
+    PlanStore store(ninst);
+    Plan pl = store.plan();
     bool want_terms = false;
     for (int i = 0; i < ninst; i++) {
         InstDesc d = make_inst(in[i].P.nqubits, in[i].k, in[i].exact, in[i].L, chunk);
         d.seed = in[i].seed;
         d.first = in[i].first;
-        d.rec_off = pl.nrec;
-        d.term_off = pl.nterms;
-        pl.nrec += in[i].count;
-        pl.nitems += in[i].count * d.nchunks;
-        pl.nterms += in[i].count * d.chi;
-        pl.rec_off.push_back(pl.nrec);
-        pl.item_off.push_back(pl.nitems);
-        pl.inst.push_back(d);
-        pl.projs.push_back(to_dev(&in[i].P));
+        pl.add(d, to_dev(&in[i].P), in[i].count, in[i].count * d.nchunks, in[i].count * d.chi);
         want_terms = want_terms || res[i].terms != nullptr;
     }
     if (pl.nrec > kMaxSamples) return fail(ctx, CSE_ERR_ARG, "%lld samples above 2^24 per call", pl.nrec);
     if (want_terms && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
     int rc = ensure_thresholds(ctx);
+    if (!rc) rc = bind_ctx_mem(ctx, pl, 0, want_terms);
     if (!rc) rc = upload_plan(ctx, pl);
-    if (!rc) rc = prep_batch(ctx, pl, 0, want_terms);
+    if (!rc) rc = prep_batch(ctx, pl, 0);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
     rc = run_records(ctx, pl, 0);
@@ -625,18 +759,16 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
 
     // the results block in one copy, into a pinned stage of the same layout
     const OutLayout o = out_layout(pl, 0);
-    rc = ensure_host(ctx, ctx->hstage, o.total);
-    if (rc) return rc;
-    char *hs = (char *)ctx->hstage.p;
+    char *hs = ctx->b_hstage;
     const long long *z = (const long long *)(hs + o.zsum);
     const double *kz = (const double *)(hs + o.kahan);
     const int32_t *se = (const int32_t *)(hs + o.sexp);
     const unsigned long long *nips = (const unsigned long long *)(hs + o.ips);
     int *errp = (int *)(hs + o.err);
-    HIPCHK(hipMemcpyAsync(hs, ctx->outb.p, o.total, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hs, ctx->b_out, o.total, hipMemcpyDeviceToHost, ctx->stream));
     for (int i = 0; i < ninst; i++)
         if (res[i].terms)
-            HIPCHK(hipMemcpyAsync(res[i].terms, dp<int32_t>(ctx->terms) + pl.inst[i].term_off,
+            HIPCHK(hipMemcpyAsync(res[i].terms, ctx->b_terms + pl.inst[i].term_off,
                                   4 * (size_t)(in[i].count * pl.inst[i].chi), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const int errv = *errp;
@@ -718,28 +850,35 @@ int cse_batch_sampled(cse_ctx *ctx, const cse_sampled_instance *inst, int ninst,
     return sampled_batch(ctx, inst, ninst, chunk, res, kernel_ms);
 }
 
+// Exact-mode instances [0, ninst) in one record launch and one sweep.  With
+// use_static the call works in this device's static scratch array and the
+// host stages hup / hstage (of at least kStaticPlanBytes / kStaticOutBytes)
+// instead of the context's buffers, and allocates nothing.
 static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chunk, cse_exact_result *res,
-                       int32_t *terms_out, double *kernel_ms) {
+                       int32_t *terms_out, double *kernel_ms, bool use_static = false, char *hup = nullptr,
+                       char *hstage = nullptr) {
     if (chunk <= 0) chunk = exact_chunk(ctx, in, ninst);
-    Plan pl;
-    pl.rec_off.push_back(0);
-    pl.item_off.push_back(0);
+    // the static path keeps its one-instance plan on the stack
+    InstDesc s_inst[1];
+    long long s_ro[2], s_io[2];
+    DevProjector s_pj[1];
+    PlanStore store(use_static ? 0 : ninst);
+    Plan pl = store.plan();
+    if (use_static) {
+        if (ninst != 1) return fail(ctx, CSE_ERR_ARG, "internal: static call with %d instances", ninst);
+        pl.inst = s_inst;
+        pl.rec_off = s_ro;
+        pl.item_off = s_io;
+        pl.projs = s_pj;
+    }
     for (int i = 0; i < ninst; i++) {
-        InstDesc d = make_inst(in[i].P.nqubits, in[i].k, in[i].exact, in[i].L, chunk);
-        d.rec_off = pl.nrec;
-        d.term_off = pl.nterms;
-        pl.nrec += d.chi;
-        pl.nitems += tri_items(d.chi, d.chunk, d.nchunks);
-        pl.nterms += d.chi * (d.chi + 1) / 2;
-        pl.rec_off.push_back(pl.nrec);
-        pl.item_off.push_back(pl.nitems);
-        pl.inst.push_back(d);
-        pl.projs.push_back(to_dev(&in[i].P));
+        const InstDesc d = make_inst(in[i].P.nqubits, in[i].k, in[i].exact, in[i].L, chunk);
+        pl.add(d, to_dev(&in[i].P), d.chi, tri_items(d.chi, d.chunk, d.nchunks), d.chi * (d.chi + 1) / 2);
     }
     if (terms_out && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
-    int rc = ensure_thresholds(ctx);
+    int rc = use_static ? bind_static_mem(ctx, pl, hup, hstage) : bind_ctx_mem(ctx, pl, 1, terms_out != nullptr);
     if (!rc) rc = upload_plan(ctx, pl);
-    if (!rc) rc = prep_batch(ctx, pl, 1, terms_out != nullptr);
+    if (!rc) rc = prep_batch(ctx, pl, 1);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
     rc = run_records(ctx, pl, 1);
@@ -752,16 +891,14 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
     // histograms (HIST_STRIDE words per instance), the Kahan pairs, the
     // inner-product counts, the error flag
     const OutLayout o = out_layout(pl, 1);
-    rc = ensure_host(ctx, ctx->hstage, o.total);
-    if (rc) return rc;
-    char *hs = (char *)ctx->hstage.p;
+    char *hs = ctx->b_hstage;
     const unsigned long long *hist = (const unsigned long long *)(hs + o.hist);
     const double *kz = (const double *)(hs + o.kahan);
     const unsigned long long *nips = (const unsigned long long *)(hs + o.ips);
     int *errp = (int *)(hs + o.err);
-    HIPCHK(hipMemcpyAsync(hs, ctx->outb.p, o.total, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hs, ctx->b_out, o.total, hipMemcpyDeviceToHost, ctx->stream));
     if (terms_out)
-        HIPCHK(hipMemcpyAsync(terms_out, ctx->terms.p, 4 * (size_t)pl.nterms, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(terms_out, ctx->b_terms, 4 * (size_t)pl.nterms, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (*errp & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
     const double kms = elapsed(ctx->e1, ctx->e2), tms = elapsed(ctx->e0, ctx->e2);
@@ -816,6 +953,30 @@ int cse_exact_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *L,
     if (!exact)
         for (int r = 0; r < k; r++) in.L[r] = L[r];
     return exact_batch(ctx, &in, 1, 0, res, nullptr, nullptr);
+}
+
+int cse_exact_projector_noalloc(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k, int exact,
+                                cse_exact_result *res) {
+    if (!ctx || !res) return CSE_ERR_ARG;
+    int rc = check_projector(ctx, P);
+    if (rc) return rc;
+    rc = check_decomp(ctx, P->nqubits, k, exact, L);
+    if (rc) return rc;
+    const int logchi = exact ? (P->nqubits + 1) / 2 : k;
+    if (logchi > kStaticMaxLogChi)
+        return fail(ctx, CSE_ERR_ARG, "chi=2^%d above the no-allocation limit 2^%d", logchi, kStaticMaxLogChi);
+    HIPCHK(hipSetDevice(ctx->device));
+    cse_instance in;
+    memset(&in, 0, sizeof(in));
+    in.P = *P;
+    in.exact = exact;
+    in.k = exact ? 0 : k;
+    if (!exact)
+        for (int r = 0; r < k; r++) in.L[r] = L[r];
+    // host stages on the stack: pageable copies, synchronized before return
+    alignas(256) char hup[kStaticPlanBytes];
+    alignas(256) char hstage[kStaticOutBytes];
+    return exact_batch(ctx, &in, 1, 0, res, nullptr, nullptr, true, hup, hstage);
 }
 
 int cse_batch_exact(cse_ctx *ctx, const cse_instance *inst, int ninst, int chunk, cse_exact_result *res,

@@ -25,7 +25,7 @@ ABI_SYMBOLS = [
     "cse_create", "cse_destroy", "cse_error", "cse_device_info", "cse_inner_products",
     "cse_prep_states", "cse_random_states", "cse_exact_projector", "cse_sampled_projector",
     "cse_batch_exact", "cse_decompose", "cse_hist_value", "cse_sampling_kept", "cse_projector_norm", "cse_projector_norms",
-    "cse_lspace_weights", "cse_batch_sampled", "cse_decompose_many",
+    "cse_lspace_weights", "cse_batch_sampled", "cse_decompose_many", "cse_exact_projector_noalloc",
 ]
 ERR_NODEV = -5
 
@@ -112,6 +112,8 @@ def lib():
                                         P(Projector), P(U64), P(ctypes.c_int32)]
         L.cse_exact_projector.argtypes = [ctypes.c_void_p, P(Projector), P(U64), ctypes.c_int, ctypes.c_int,
                                           P(ExactResult)]
+        L.cse_exact_projector_noalloc.argtypes = [ctypes.c_void_p, P(Projector), P(U64), ctypes.c_int,
+                                                  ctypes.c_int, P(ExactResult)]
         L.cse_sampled_projector.argtypes = [ctypes.c_void_p, P(Projector), P(U64), ctypes.c_int, ctypes.c_int,
                                             U64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, P(SampledResult)]
         L.cse_batch_exact.argtypes = [ctypes.c_void_p, P(Instance), ctypes.c_int, ctypes.c_int, P(ExactResult),
@@ -215,6 +217,19 @@ class Engine:
             r.hist = _ptr(hist, ctypes.c_int64)
         self._check(self._L.cse_exact_projector(self.h, ctypes.byref(proj), _ptr(L, U64), k, int(exact),
                                                 ctypes.byref(r)))
+        return r, hist
+
+    def exact_projector_noalloc(self, proj, L, k, exact, want_hist=False):
+        """exact_projector through cse_exact_projector_noalloc: the same
+        result, computed in the library's static device array (chi <= 2^13)."""
+        L = np.ascontiguousarray(np.array(list(L) + [0] * (NMAX - len(L)), dtype=np.uint64))
+        r = ExactResult()
+        hist = None
+        if want_hist:
+            hist = np.zeros(HIST_BINS, dtype=np.int64)
+            r.hist = _ptr(hist, ctypes.c_int64)
+        self._check(self._L.cse_exact_projector_noalloc(self.h, ctypes.byref(proj), _ptr(L, U64), k, int(exact),
+                                                        ctypes.byref(r)))
         return r, hist
 
     def sampled_projector(self, proj, L, k, exact, seed, first, count, chunk=0, want_terms=False,

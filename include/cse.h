@@ -58,6 +58,7 @@ const char *cse_error(const cse_ctx *ctx);
 /* number of compute units and the device name, for reports */
 int cse_device_info(cse_ctx *ctx, int *cus, char *name, int name_len);
 
+
 /*
  * <s2[i] | s1[i]> for npairs state pairs of dimension n: one wavefront per
  * pair.  Replaces innerProductExact (libcirc/stabilizer/stabilizer.h:29,
@@ -119,6 +120,21 @@ typedef struct {
  */
 int cse_exact_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k,
                         int exact, cse_exact_result *res);
+
+/*
+ * cse_exact_projector without any allocation.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls
+ * no hipMalloc, hipFree, host allocator or other allocating routine: every
+ * device buffer the call needs (work queue, plan, results block, records)
+ * lives in a static device array of 8,761,856 bytes that each GPU holds
+ * until the process ends, and the host stages are on the stack.  Results
+ * equal cse_exact_projector's.  chi <= 2^13 (|H^t> with t <= 26, or k <= 13);
+ * larger chi returns CSE_ERR_ARG.  Every call on one GPU shares that GPU's
+ * array, from any context: calls that overlap on one GPU race.
+ */
+int cse_exact_projector_noalloc(cse_ctx *ctx, const cse_projector *P, const uint64_t *L, int k,
+                                int exact, cse_exact_result *res);
 
 typedef struct {
     double mean;        /* mean over samples of 2^t |pf * sum_i <phi_i|theta'>|^2,
