@@ -1,21 +1,30 @@
-"""Item-size autotuner for the engine's persistent sweep.
+"""Autotuner for the engine's persistent sweep: the knobs that change how the
+work is cut and which kernel runs it, never the results.
 
-The sweep's device work queue hands out (record, chunk of j) items.  The
-chunk size trades per-item cost (the record load, the lane tables, the flush
-of the histogram or of the per-record sums) against balance: a launch ends
-when the last resident wave finishes its last item.  Every chunk size gives
-the same terms and the same exact results (integer sums, independent of
-order), so the choice is a timing question only.
+Exact mode (cse_batch_exact): the sweep's occupancy and the claim cap.
+Kernel 1 runs the exact sweep at seven waves per SIMD, kernel 2 at eight
+(more latency hidden, more SGPRs spilled to VGPR lanes): large mixed
+launches gain a few percent at eight, launches of many small instances lose
+a few.  The sweep hands out slices of 64 j, a static first claim per wave
+and then guided self-scheduling from eight counters; `chunk` caps a claim.
+Larger caps cut the per-segment cost (the record load, the lane tables) on
+launches with many slices per wave; the guided claims shrink at the end of
+the queue whatever the cap.
 
-tune_exact_chunk / tune_sampled_chunk time each candidate on the device (the
+Sampled mode (cse_batch_sampled): the sweep kernel and the item size.  Kernel
+1 is the eight-wave sweep whose hand-scheduled path takes records of at most
+two non-pivot columns (others run compiled code), kernel 2 the eight-wave
+sweep with the general restriction loop (cse_set_sweep_kernel); the item size
+trades per-item cost (record load, tables, flush of the per-record sums)
+against balance.
+
+Every setting gives the same terms and the same exact results (integer sums,
+independent of order), so the choice is a timing question only.
+tune_exact / tune_sampled time each candidate on the device (the
 sweep's HIP-event time, best of `reps`) on a proxy of the workload, together
-with the engine's own size rule (chunk 0), and return the fastest, cached per
-(device, kind, shape key) for the process.  The engine's rule stays (0 is
-returned) unless a fixed size beats it by more than `tol`: on large launches
-every size lands within about 1-2% and the proxy ranks them within about 2%
-of the whole launch, while small launches (a few items per resident wave)
-differ by 5-50% (measured on MI355X: 2,048 records of one projector run 5%
-faster in items of 192 than of 256, and 70% slower in items of 1024).
+with the engine's own rule, and return the fastest, cached per (device, kind,
+shape key) for the process.  The engine's rule stays unless a candidate
+beats it by more than `tol`.
 
 Proxy.  A workload with many items per resident wave is throughput-bound, and
 the candidates rank on a part of it as on the whole: exact batches keep their
@@ -29,8 +38,13 @@ import threading
 
 import numpy as np
 
-# item sizes in j (terms); 64 terms run per asm block, so multiples of 64
+# sampled mode: item sizes in j (terms); 64 terms run per asm block, so
+# multiples of 64
 CANDIDATES = (128, 192, 256, 384, 512, 1024)
+# the sweep kernels of either mode (cse_set_sweep_kernel)
+KERNELS = (1, 2)
+# exact mode: claim caps in j (multiples of the 64-j slice)
+EXACT_CAPS = (256, 512, 1024, 2048, 4096)
 # resident waves per CU of the two packed sweep kernels (4 waves per block)
 _WAVES_PER_CU = {"exact": 7 * 4, "sampled": 8 * 4}
 # packed layout (D in the J rows) up to this width
@@ -74,19 +88,23 @@ def _key(E, kind, ts, hdec, chis, items):
     return (name, cus, kind, packed, bool(hdec), load, int(round(math.log2(mean_chi))))
 
 
-def _best(run, candidates, reps, tol):
-    timings = {c: min(run(c) for _ in range(reps)) for c in (0,) + tuple(candidates)}
+def _best(run, candidates, reps, tol, rule=0):
+    """the fastest of candidates, or `rule` (the engine's own choice, timed
+    first) unless a candidate beats it by more than tol"""
+    timings = {c: min(run(c) for _ in range(reps)) for c in (rule,) + tuple(candidates)}
     best = min(timings, key=timings.get)
-    if timings[0] <= timings[best] * (1.0 + tol):
-        best = 0
+    if timings[rule] <= timings[best] * (1.0 + tol):
+        best = rule
     return best, timings
 
 
-def tune_exact_chunk(E, instances, candidates=CANDIDATES, reps=2, items_per_wave=32, tol=0.03, timings=None):
-    """The item size for cse_batch_exact on these instances (a list of
-    Instance or an instance_array): a candidate, or 0 for the engine's rule.
-    `timings`, if a dict, receives the sweep milliseconds of each candidate
-    (0 included) on the proxy."""
+def tune_exact(E, instances, kernels=KERNELS, candidates=EXACT_CAPS, reps=2, items_per_wave=32, tol=0.02,
+               timings=None):
+    """(kernel, claim cap in j) for cse_batch_exact on these instances (a
+    list of Instance or an instance_array): (0, 0) for the engine's rule,
+    else a kernel of `kernels` with a cap of (0,) + `candidates`.
+    `timings`, if a dict, receives the sweep milliseconds of each (kernel,
+    cap) on the proxy."""
     from .engine import INSTANCE_DTYPE, Instance
     fields = _exact_fields(instances)
     if not fields:
@@ -117,11 +135,18 @@ def tune_exact_chunk(E, instances, candidates=CANDIDATES, reps=2, items_per_wave
     else:
         proxy = [instances[i] for i in pick]
         assert all(isinstance(p, Instance) for p in proxy)
+    keep = getattr(E, "sweep_kernels", [0, 0])[1]
 
-    def run(c):
-        return E.batch_exact(proxy, chunk=c)[2]
+    def run(kc):
+        kernel, chunk = kc
+        E.set_sweep_kernel(kernel, mode=1)
+        try:
+            return E.batch_exact(proxy, chunk=chunk)[2]
+        finally:
+            E.set_sweep_kernel(keep, mode=1)
 
-    best, t = _best(run, candidates, reps, tol)
+    cands = [(kern, c) for kern in kernels for c in (0,) + tuple(candidates)]
+    best, t = _best(run, cands, reps, tol, rule=(0, 0))
     if timings is not None:
         timings.update(t)
     with _lock:
@@ -129,12 +154,14 @@ def tune_exact_chunk(E, instances, candidates=CANDIDATES, reps=2, items_per_wave
     return best
 
 
-def tune_sampled_chunk(E, instances, candidates=CANDIDATES, reps=2, items_per_wave=32, tol=0.03,
-                       timings=None):
-    """The item size for cse_batch_sampled on these instances (a
-    sampled_instance_array), or 0 for the engine's rule; the proxy keeps each
-    instance's seed and first sample and a proportional share of its
-    count."""
+def tune_sampled(E, instances, kernels=KERNELS, candidates=CANDIDATES, reps=2, items_per_wave=32, tol=0.02,
+                 timings=None):
+    """(kernel, item size) for cse_batch_sampled on these instances (a
+    sampled_instance_array): (0, 0) for the engine's rule, else a kernel of
+    `kernels` with an item size of (0,) + `candidates`.  The proxy keeps
+    each instance's seed and first sample and a proportional share of its
+    count.  `timings`, if a dict, receives the sweep milliseconds of each
+    (kernel, size) on the proxy."""
     if len(instances) == 0:
         raise ValueError("no instances")
     ts = [int(t) for t in instances["P"]["nqubits"]]
@@ -151,11 +178,18 @@ def tune_sampled_chunk(E, instances, candidates=CANDIDATES, reps=2, items_per_wa
     if total > target:
         share = target / total
         proxy["count"] = [max(1, int(math.ceil(n * share))) for n in counts]
+    keep = getattr(E, "sweep_kernels", [0, 0])[0]
 
-    def run(c):
-        return E.batch_sampled(proxy, chunk=c)[2]
+    def run(kc):
+        kernel, chunk = kc
+        E.set_sweep_kernel(kernel, mode=0)
+        try:
+            return E.batch_sampled(proxy, chunk=chunk)[2]
+        finally:
+            E.set_sweep_kernel(keep, mode=0)
 
-    best, t = _best(run, candidates, reps, tol)
+    cands = [(kern, c) for kern in kernels for c in (0,) + tuple(candidates)]
+    best, t = _best(run, cands, reps, tol, rule=(0, 0))
     if timings is not None:
         timings.update(t)
     with _lock:

@@ -31,6 +31,22 @@ struct HostBuf {
     size_t cap = 0;
 };
 
+// What a captured batch depends on besides the plan's and the results'
+// contents: its sizes, kernels and grids (all functions of these) and every
+// buffer address its nodes hold.  A call whose key matches replays the graph.
+struct GraphKey {
+    int mode, ninst, packed, kernel;
+    long long nrec, nitems, take_max;
+    size_t plan_bytes, out_bytes;
+    const void *ptr[8];
+};
+
+struct GraphEntry {
+    GraphKey key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long used = 0;
+};
+
 struct cse_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -62,12 +78,17 @@ struct cse_ctx {
     uint64_t *b_recs = nullptr;
     unsigned long long *b_queue = nullptr;
     int32_t *b_sexp1 = nullptr, *b_terms = nullptr;
+    // batched calls replayed from captured HIP graphs (cse_set_graphs)
+    bool use_graphs = false;
+    std::vector<GraphEntry> graphs;
+    unsigned long long graph_clock = 0;
     // this device's copy of the static scratch array (nullptr if its address
     // could not be obtained; scratch_err says why)
     char *scratch = nullptr;
     char scratch_err[160] = {0};
     bool thr_ready = false;
     int blocks_per_cu = 8;  // persistent grid: blocks of 4 waves per CU
+    int sweep_kernel[2] = {0, 0};  // per mode (cse_set_sweep_kernel)
 };
 
 // ------------------------------------------------- static scratch array
@@ -82,11 +103,11 @@ constexpr long long kStaticMaxChi = 1ll << kStaticMaxLogChi;
 constexpr size_t kStaticPlanBytes = 4096;  // one instance's plan: 2,816 B
 constexpr size_t kStaticOutBytes = 8448;   // one instance's results block: 8,288 B
 constexpr size_t kScratchQueue = 0;
-constexpr size_t kScratchPlan = 256;
+constexpr size_t kScratchPlan = 8 * CSE_QUEUE_WORDS;
 constexpr size_t kScratchOut = kScratchPlan + kStaticPlanBytes;
 constexpr size_t kScratchSexp = kScratchOut + kStaticOutBytes;
 constexpr size_t kScratchRecs = kScratchSexp + 4 * (size_t)kStaticMaxChi;
-constexpr size_t kScratchBytes = kScratchRecs + (size_t)REC_WORDS * 8 * (size_t)kStaticMaxChi;  // 8,761,856 B
+constexpr size_t kScratchBytes = kScratchRecs + (size_t)REC_WORDS * 8 * (size_t)kStaticMaxChi;  // 8,762,624 B
 static_assert(kScratchOut % 256 == 0 && kScratchSexp % 256 == 0 && kScratchRecs % 256 == 0, "scratch parts aligned");
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -235,10 +256,11 @@ int default_chunk(long long chi, int size = 256) {
     return (int)(c < 1 ? 1 : c);
 }
 
-// Exact mode's default item size: 512 j when the launch still has eight
-// items per resident wave of k_sweep_packed_all at that size (seven waves per
-// SIMD), else 256.  Larger items halve the per-item record load, table set-up
-// and histogram flush; smaller ones keep every wave busy on small launches.
+// Exact mode's default largest claim: 512 j when the launch still has eight
+// items of that size per resident wave of k_sweep_packed_all (seven waves per
+// SIMD), else 256.  Larger claims halve the per-segment record load, table
+// set-up and histogram flush; the guided self-scheduling of the sweep shrinks
+// the claims at the end of the queue in any case.
 int exact_chunk(cse_ctx *ctx, const cse_instance *in, int ninst) {
     long long items = 0;
     for (int i = 0; i < ninst; i++) {
@@ -247,7 +269,7 @@ int exact_chunk(cse_ctx *ctx, const cse_instance *in, int ninst) {
         const long long per = (chi + 511) / 512;
         items += chi * (per + 1) / 2;
     }
-    const long long waves = (long long)ctx->cus * 7 * 4;
+    const long long waves = (long long)ctx->cus * CSE_EXACT_BLOCKS_PER_CU * 4;
     return items >= 8 * waves ? 512 : 256;
 }
 
@@ -268,6 +290,7 @@ struct Plan {
     long long *item_off = nullptr;  // ninst + 1
     DevProjector *projs = nullptr;
     long long nrec = 0, nitems = 0, nterms = 0;
+    long long take_max = 1;  // exact mode: largest claim, in slices of 64 j
     // appends an instance of nr records, ni work items and nt terms (its
     // record and term offsets are set here)
     void add(InstDesc d, const DevProjector &p, long long nr, long long ni, long long nt) {
@@ -318,21 +341,26 @@ size_t plan_layout(const Plan &pl, size_t off[4], size_t sz[4]) {
     return total;
 }
 
-// The plan goes up from the bound host stage in one transfer, into the bound
-// device plan of the same layout; every call synchronizes its stream before
-// returning, so the stage is free again at the next call.
-int upload_plan(cse_ctx *ctx, const Plan &pl) {
+// The plan is written into the bound host stage in its device layout, and
+// the device pointers into the bound plan are set; copy_plan then moves it
+// in one transfer.  Every call synchronizes its stream before returning, so
+// the stage is free again at the next call.
+size_t stage_plan(cse_ctx *ctx, const Plan &pl) {
     size_t off[4], sz[4];
     const size_t total = plan_layout(pl, off, sz);
     const void *src[4] = {pl.inst, pl.item_off, pl.rec_off, pl.projs};
     char *h = ctx->b_hup;
     for (int b = 0; b < 4; b++) memcpy(h + off[b], src[b], sz[b]);
-    HIPCHK(hipMemcpyAsync(ctx->b_plan, h, total, hipMemcpyHostToDevice, ctx->stream));
     char *d = ctx->b_plan;
     ctx->d_inst = (InstDesc *)(d + off[0]);
     ctx->d_itemoff = (long long *)(d + off[1]);
     ctx->d_recoff = (long long *)(d + off[2]);
     ctx->d_projs = (DevProjector *)(d + off[3]);
+    return total;
+}
+
+int copy_plan(cse_ctx *ctx, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(ctx->b_plan, ctx->b_hup, bytes, hipMemcpyHostToDevice, ctx->stream));
     return CSE_OK;
 }
 
@@ -389,7 +417,7 @@ int bind_ctx_mem(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     ENSURE(plan, pbytes);
     ENSURE(outb, o.total);
     ENSURE(recs, (size_t)REC_WORDS * 8 * (size_t)pl.nrec);
-    ENSURE(queue, 8);
+    ENSURE(queue, 8 * CSE_QUEUE_WORDS);
     if (mode == 1) ENSURE(sexp, 4 * (size_t)pl.nrec);
     if (want_terms) ENSURE(terms, 4 * (size_t)(pl.nterms > 0 ? pl.nterms : 1));
     int rc = ensure_host(ctx, ctx->hup, pbytes);
@@ -444,7 +472,7 @@ int prep_batch(cse_ctx *ctx, const Plan &pl, int mode) {
     ZeroList zl;
     memset(&zl, 0, sizeof(zl));
     zl_add(zl, b, (long long)(o.total / 8));
-    zl_add(zl, ctx->b_queue, 1);
+    zl_add(zl, ctx->b_queue, CSE_QUEUE_WORDS);
     return zero_list(ctx, zl);
 }
 
@@ -470,11 +498,111 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     // exact mode takes k_sweep_packed_all (seven waves per SIMD); sampled
     // launches, |L> and |H^t> terms alike, take k_sweep_packed (eight)
     const bool all = packed && mode == 1;
-    const int grid = sweep_grid(ctx, pl.nitems, all ? 7 : ctx->blocks_per_cu);
-    if (all) hipLaunchKernelGGL(k_sweep_packed_all, dim3(grid), dim3(256), 0, ctx->stream, A);
+    const bool all8 = all && ctx->sweep_kernel[1] == 2;
+    const int grid = sweep_grid(ctx, pl.nitems, all8 ? 8 : all ? CSE_EXACT_BLOCKS_PER_CU : ctx->blocks_per_cu);
+    A.take_max = pl.take_max;
+    // the static first claims cover about half of the slices (at most
+    // take_max per wave); the rest is cut into CSE_NQUEUES parts
+    const long long waves = 4ll * grid;
+    A.first_take = pl.nitems / (2 * waves);
+    if (A.first_take > A.take_max) A.first_take = A.take_max;
+    if (A.first_take < 1) A.first_take = 1;
+    A.static_end = A.first_take * waves < pl.nitems ? A.first_take * waves : pl.nitems;
+    A.part = (pl.nitems - A.static_end + CSE_NQUEUES - 1) / CSE_NQUEUES;
+    A.waves_per_part = (waves + CSE_NQUEUES - 1) / CSE_NQUEUES;
+    if (all8) hipLaunchKernelGGL(k_sweep_packed_all8, dim3(grid), dim3(256), 0, ctx->stream, A);
+    else if (all) hipLaunchKernelGGL(k_sweep_packed_all, dim3(grid), dim3(256), 0, ctx->stream, A);
+    else if (packed && ctx->sweep_kernel[0] == 2)
+        hipLaunchKernelGGL(k_sweep_packed_gen, dim3(grid), dim3(256), 0, ctx->stream, A);
     else if (packed) hipLaunchKernelGGL(k_sweep_packed, dim3(grid), dim3(256), 0, ctx->stream, A);
     else hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(256), 0, ctx->stream, A);
     HIPCHK(hipGetLastError());
+    return CSE_OK;
+}
+
+// The device work of a bound, staged batch in stream order: plan copy,
+// clear, records, sweep, results copy; with events between the launches
+// when timed.
+int enqueue_batch(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms, size_t plan_bytes, bool timed) {
+    int rc = copy_plan(ctx, plan_bytes);
+    if (!rc) rc = prep_batch(ctx, pl, mode);
+    if (rc) return rc;
+    if (timed) HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
+    rc = run_records(ctx, pl, mode);
+    if (rc) return rc;
+    if (timed) HIPCHK(hipEventRecord(ctx->e1, ctx->stream));
+    rc = run_sweep(ctx, pl, mode, want_terms);
+    if (rc) return rc;
+    if (timed) HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->b_hstage, ctx->b_out, ctx->out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    return CSE_OK;
+}
+
+constexpr size_t kMaxGraphs = 16;
+
+// Runs a bound, staged batch: eagerly, or -- when the context replays graphs
+// and the call writes no per-term output -- from the context's graph of the
+// same key, captured at the first call of that key.  *graphed tells which;
+// the caller synchronizes the stream.
+int run_batch(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms, size_t plan_bytes, bool *graphed) {
+    *graphed = false;
+    if (!ctx->use_graphs || want_terms) return enqueue_batch(ctx, pl, mode, want_terms, plan_bytes, true);
+    GraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.mode = mode;
+    key.ninst = pl.ninst;
+    key.packed = 1;
+    for (int i = 0; i < pl.ninst; i++) key.packed &= pl.inst[i].n <= CSE_END_ROW ? 1 : 0;
+    key.kernel = ctx->sweep_kernel[mode];
+    key.nrec = pl.nrec;
+    key.nitems = pl.nitems;
+    key.take_max = pl.take_max;
+    key.plan_bytes = plan_bytes;
+    key.out_bytes = out_layout(pl, mode).total;
+    const void *ptr[8] = {ctx->b_plan, ctx->b_out, ctx->b_recs, ctx->b_queue,
+                          ctx->b_sexp1, ctx->b_hup, ctx->b_hstage, mode == 0 ? ctx->thr.p : nullptr};
+    memcpy(key.ptr, ptr, sizeof(ptr));
+    GraphEntry *hit = nullptr;
+    for (GraphEntry &g : ctx->graphs)
+        if (memcmp(&g.key, &key, sizeof(key)) == 0) {
+            hit = &g;
+            break;
+        }
+    if (!hit) {
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_batch(ctx, pl, mode, false, plan_bytes, false);
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        if (rc || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            return rc ? rc : fail(ctx, CSE_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(ctx, CSE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        }
+        if (ctx->graphs.size() >= kMaxGraphs) {
+            size_t old = 0;
+            for (size_t g = 1; g < ctx->graphs.size(); g++)
+                if (ctx->graphs[g].used < ctx->graphs[old].used) old = g;
+            (void)hipGraphExecDestroy(ctx->graphs[old].exec);
+            ctx->graphs.erase(ctx->graphs.begin() + (long)old);
+        }
+        GraphEntry ent;
+        ent.key = key;
+        ent.exec = exec;
+        ctx->graphs.push_back(ent);
+        hit = &ctx->graphs.back();
+    }
+    hit->used = ++ctx->graph_clock;
+    HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
+    HIPCHK(hipGraphLaunch(hit->exec, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
+    *graphed = true;
     return CSE_OK;
 }
 
@@ -599,6 +727,9 @@ void cse_destroy(cse_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (GraphEntry &g : ctx->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    ctx->graphs.clear();
     DevBuf *bufs[] = {&ctx->s1, &ctx->s2, &ctx->out, &ctx->recs, &ctx->terms, &ctx->queue, &ctx->thr,
                       &ctx->sexp, &ctx->L, &ctx->idx, &ctx->errflag, &ctx->lsh, &ctx->plan, &ctx->outb};
     for (DevBuf *b : bufs)
@@ -615,6 +746,20 @@ void cse_destroy(cse_ctx *ctx) {
 const char *cse_error(const cse_ctx *ctx) {
     if (ctx) return ctx->err;
     return g_create_err[0] ? g_create_err : "no context";
+}
+
+int cse_set_sweep_kernel(cse_ctx *ctx, int mode, int kernel) {
+    if (!ctx) return CSE_ERR_ARG;
+    if (mode < 0 || mode > 1) return fail(ctx, CSE_ERR_ARG, "sweep mode %d outside [0, 1]", mode);
+    if (kernel < 0 || kernel > 2) return fail(ctx, CSE_ERR_ARG, "sweep kernel %d outside [0, 2]", kernel);
+    ctx->sweep_kernel[mode] = kernel;
+    return CSE_OK;
+}
+
+int cse_set_graphs(cse_ctx *ctx, int enable) {
+    if (!ctx) return CSE_ERR_ARG;
+    ctx->use_graphs = enable != 0;
+    return CSE_OK;
 }
 
 int cse_device_info(cse_ctx *ctx, int *cus, char *name, int name_len) {
@@ -746,18 +891,13 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
     if (want_terms && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
     int rc = ensure_thresholds(ctx);
     if (!rc) rc = bind_ctx_mem(ctx, pl, 0, want_terms);
-    if (!rc) rc = upload_plan(ctx, pl);
-    if (!rc) rc = prep_batch(ctx, pl, 0);
     if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
-    rc = run_records(ctx, pl, 0);
+    bool graphed = false;
+    rc = run_batch(ctx, pl, 0, want_terms, stage_plan(ctx, pl), &graphed);
     if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->e1, ctx->stream));
-    rc = run_sweep(ctx, pl, 0, want_terms);
-    if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
 
-    // the results block in one copy, into a pinned stage of the same layout
+    // the results block came back in one copy, into a pinned stage of the
+    // same layout
     const OutLayout o = out_layout(pl, 0);
     char *hs = ctx->b_hstage;
     const long long *z = (const long long *)(hs + o.zsum);
@@ -765,7 +905,6 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
     const int32_t *se = (const int32_t *)(hs + o.sexp);
     const unsigned long long *nips = (const unsigned long long *)(hs + o.ips);
     int *errp = (int *)(hs + o.err);
-    HIPCHK(hipMemcpyAsync(hs, ctx->b_out, o.total, hipMemcpyDeviceToHost, ctx->stream));
     for (int i = 0; i < ninst; i++)
         if (res[i].terms)
             HIPCHK(hipMemcpyAsync(res[i].terms, ctx->b_terms + pl.inst[i].term_off,
@@ -774,7 +913,9 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
     const int errv = *errp;
     if (errv & 1) return fail(ctx, CSE_ERR_RNG, "random state draw exceeded its shrink cap");
     if (errv & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
-    const double pms = elapsed(ctx->e0, ctx->e1), kms = elapsed(ctx->e1, ctx->e2);
+    // a replayed graph is timed as a whole (records and sweep together)
+    const double pms = graphed ? 0.0 : elapsed(ctx->e0, ctx->e1);
+    const double kms = graphed ? elapsed(ctx->e0, ctx->e2) : elapsed(ctx->e1, ctx->e2);
     if (kernel_ms) *kernel_ms = kms;
     auto convert = [&](int lo, int hi) {
         for (int i = lo; i < hi; i++) {
@@ -871,37 +1012,37 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
         pl.item_off = s_io;
         pl.projs = s_pj;
     }
+    // items are slices of CSE_SLICE_J j; a claim takes at most chunk / CSE_SLICE_J of them
+    pl.take_max = chunk / CSE_SLICE_J > 1 ? chunk / CSE_SLICE_J : 1;
     for (int i = 0; i < ninst; i++) {
-        const InstDesc d = make_inst(in[i].P.nqubits, in[i].k, in[i].exact, in[i].L, chunk);
+        const InstDesc d = make_inst(in[i].P.nqubits, in[i].k, in[i].exact, in[i].L, CSE_SLICE_J);
         pl.add(d, to_dev(&in[i].P), d.chi, tri_items(d.chi, d.chunk, d.nchunks), d.chi * (d.chi + 1) / 2);
     }
     if (terms_out && pl.nterms > (1ll << 31)) return fail(ctx, CSE_ERR_ARG, "per-term output too large");
     int rc = use_static ? bind_static_mem(ctx, pl, hup, hstage) : bind_ctx_mem(ctx, pl, 1, terms_out != nullptr);
-    if (!rc) rc = upload_plan(ctx, pl);
-    if (!rc) rc = prep_batch(ctx, pl, 1);
     if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->e0, ctx->stream));
-    rc = run_records(ctx, pl, 1);
+    // the static path never captures (graph memory nodes would be
+    // allocations of their own)
+    bool graphed = false;
+    const size_t plan_bytes = stage_plan(ctx, pl);
+    rc = use_static ? enqueue_batch(ctx, pl, 1, false, plan_bytes, true)
+                    : run_batch(ctx, pl, 1, terms_out != nullptr, plan_bytes, &graphed);
     if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->e1, ctx->stream));
-    rc = run_sweep(ctx, pl, 1, terms_out != nullptr);
-    if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->e2, ctx->stream));
-    // the results block in one copy, into a pinned stage of the same layout:
-    // histograms (HIST_STRIDE words per instance), the Kahan pairs, the
-    // inner-product counts, the error flag
+    // the results block came back in one copy, into a pinned stage of the
+    // same layout: histograms (HIST_STRIDE words per instance), the Kahan
+    // pairs, the inner-product counts, the error flag
     const OutLayout o = out_layout(pl, 1);
     char *hs = ctx->b_hstage;
     const unsigned long long *hist = (const unsigned long long *)(hs + o.hist);
     const double *kz = (const double *)(hs + o.kahan);
     const unsigned long long *nips = (const unsigned long long *)(hs + o.ips);
     int *errp = (int *)(hs + o.err);
-    HIPCHK(hipMemcpyAsync(hs, ctx->b_out, o.total, hipMemcpyDeviceToHost, ctx->stream));
     if (terms_out)
         HIPCHK(hipMemcpyAsync(terms_out, ctx->b_terms, 4 * (size_t)pl.nterms, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (*errp & 2) return fail(ctx, CSE_ERR_HIP, "internal: an inner product exponent left [-64, 0]");
-    const double kms = elapsed(ctx->e1, ctx->e2), tms = elapsed(ctx->e0, ctx->e2);
+    const double tms = elapsed(ctx->e0, ctx->e2);
+    const double kms = graphed ? tms : elapsed(ctx->e1, ctx->e2);
     if (kernel_ms) *kernel_ms = kms;
     // each instance's exact value from its histogram (a few microseconds
     // each): split over host threads for large batches

@@ -38,6 +38,24 @@ inline __host__ __device__ long long tri_items(long long chi, long long ch, long
     return tri_before(ch, nch, nch - 1) + (chi - (nch - 1) * ch);
 }
 
+// Exact-mode work items: slices of CSE_SLICE_J consecutive j of one record.
+#ifndef CSE_SLICE_J
+#define CSE_SLICE_J 64
+#endif
+
+// Exact-mode work queue: counters, one per part of the dynamic range, 128
+// bytes (16 words) apart, so a launch's claims spread over the XCDs'
+// memory channels instead of one address.
+#define CSE_NQUEUES 8
+#define CSE_QUEUE_STRIDE 16
+#define CSE_QUEUE_WORDS (CSE_NQUEUES * CSE_QUEUE_STRIDE)
+
+// Resident blocks (of 4 waves) per CU of the exact-mode sweep
+// k_sweep_packed_all: its launch bound and its persistent grid.
+#ifndef CSE_EXACT_BLOCKS_PER_CU
+#define CSE_EXACT_BLOCKS_PER_CU 7
+#endif
+
 // Histogram bins per instance: P in [-(64 + 64), 0], 8 phases (the public
 // CSE_HIST_BINS).
 #define HIST_STRIDE (129 * 8)
@@ -48,7 +66,7 @@ struct InstDesc {
     int n;              // state dimension t
     int k;              // rows of L (exact == 0)
     int exact;          // 1: |H^t> decomposition, 0: |L>
-    int chunk;          // j per work item
+    int chunk;          // j per work item (exact-mode batches: per slice, 64)
     long long chi;      // decomposition size
     long long nchunks;  // work items per record
     long long rec_off;  // first record of this instance
@@ -63,6 +81,17 @@ struct SweepArgs {
     int ninst;
     long long nitems;
     const long long *item_off;  // ninst + 1 prefix of items
+    // mode 1: items are slices of up to 64 j (InstDesc.chunk).  Global wave
+    // g first takes slices [g first_take, g first_take + first_take) of
+    // [0, static_end); the rest are cut into CSE_NQUEUES parts of `part`
+    // slices, each handed out from its own counter (queue + q
+    // CSE_QUEUE_STRIDE) by guided self-scheduling: a claim takes remaining /
+    // waves_per_part slices, at least 1 and at most take_max
+    long long take_max;
+    long long first_take;
+    long long static_end;
+    long long part;
+    long long waves_per_part;
     const InstDesc *inst;
     const uint64_t *recs;
     unsigned long long *queue;

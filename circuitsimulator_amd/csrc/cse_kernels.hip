@@ -1150,9 +1150,10 @@ DEV double wave_sumf64(double v) {
 
 // Fold the parked terms (lanes < cnt).  Packed term: 0 for eps = 0, else
 // 1 + ((-p) * 8 + m) * 2; in mode 1 the term in lane diag is the diagonal
-// one and every other term counts as 2 Re.  Returns 1 in a lane whose term
-// is out of range (not folded).
-DEV int fold_terms(int packed, int cnt, int mode, int diag, int sexp, LaneAcc &acc, unsigned int *hist) {
+// one and every other term counts as 2 Re, into histogram bin hoff +
+// (-p) * 8 + m.  Returns 1 in a lane whose term is out of range (not folded).
+DEV int fold_terms(int packed, int cnt, int mode, int diag, int sexp, LaneAcc &acc, unsigned int *hist,
+                   int hoff) {
     const int x = lane_id();
     if (x >= cnt || packed == 0) return 0;
     const int v = packed - 1;
@@ -1171,11 +1172,11 @@ DEV int fold_terms(int packed, int cnt, int mode, int diag, int sexp, LaneAcc &a
         term_f32(p - sexp, m, vr, vi);
         if (off) {
             acc.kahan(2.f * vr, 0.f);
-            atomicAdd(&hist[pm], 1u);
-            atomicAdd(&hist[(pm & ~7) | ((8 - m) & 7)], 1u);
+            atomicAdd(&hist[hoff + pm], 1u);
+            atomicAdd(&hist[hoff + ((pm & ~7) | ((8 - m) & 7))], 1u);
         } else {
             acc.kahan(vr, vi);
-            atomicAdd(&hist[pm], 1u);
+            atomicAdd(&hist[hoff + pm], 1u);
         }
     }
     return 0;
@@ -1186,39 +1187,127 @@ DEV int fold_terms(int packed, int cnt, int mode, int diag, int sexp, LaneAcc &a
 // Persistent sweep.  mode 0 (sampled): record r is theta_s, all j in [0, chi)
 // contribute to per-record Z[w] sums.  mode 1 (exact): record r is theta'_i
 // of its instance, j in [i, chi), weight 1 on i == j and 2 Re otherwise, into
-// the instance's (P, M) histogram.
-template <bool PK, bool ALL>
+// the instance's (P, M) histogram.  MODE is the mode when fixed at compile
+// time (0 or 1), -1 for A.mode.  With MODE == 1 a wave keeps its partial
+// histogram (over P = p - s, so across records) and its Kahan sums from
+// segment to segment of one instance and flushes them to the instance's
+// global sums when it moves to another instance and at its exit: one set of
+// global atomics per (wave, instance) instead of per segment.
+template <bool PK, bool ALL, int MODE>
 DEV void sweep_body(const SweepArgs &A) {
-    __shared__ unsigned int lhist[4][LH_BINS];
+    constexpr bool PERSIST = MODE == 1;
+    constexpr int HB = MODE == 0 ? 64 : (PERSIST ? HIST_STRIDE : LH_BINS);
+    __shared__ unsigned int lhist[4][HB];
+    const int mode = MODE >= 0 ? MODE : A.mode;
     const int wv = (int)(threadIdx.x >> 6);
     const int x = lane_id();
     unsigned int *hist = lhist[wv];
-    for (int b = x; b < LH_BINS; b += 64) hist[b] = 0;
+    for (int b = x; b < HB; b += 64) hist[b] = 0;
     const u64 lowx = lowmask(x);
+    // PERSIST: the instance whose partial sums the wave holds, and those sums
+    int cur = -1;
+    LaneAcc pacc;
+    pacc.clear();
+    auto flush_instance = [&]() {
+        if (cur < 0) return;
+        __threadfence_block();
+        const double kr = wave_sumf64((double)pacc.sr - (double)pacc.cr);
+        const double ki = wave_sumf64((double)pacc.si - (double)pacc.ci);
+        if (x == 0) {
+            double *kz = A.kahan + 2 * (size_t)cur;
+            atomicAdd(&kz[0], kr);
+            atomicAdd(&kz[1], ki);
+        }
+        for (int b = x; b < HB; b += 64) {
+            const unsigned int v = hist[b];
+            if (v) {
+                atomicAdd((unsigned long long *)&A.hist[(size_t)cur * A.hist_stride + b], (unsigned long long)v);
+                hist[b] = 0;
+            }
+        }
+        pacc.clear();
+    };
 
+    // mode 1: the wave's claimed slice range [s_cur, s_end).  Its first claim
+    // is static, slices [g f0, g f0 + f0) of the first static_end for global
+    // wave g, so the waves start without queuing on a counter.  The rest,
+    // [static_end, nitems), is cut into CSE_NQUEUES equal parts, each with
+    // its own counter (128 bytes apart); a wave claims from the part of its
+    // block's XCD first (blocks go to the XCDs round robin) and moves on to
+    // the next part when that one is drained, so the claims of the whole
+    // grid spread over the counters.  Claims are sized by guided
+    // self-scheduling from the position the wave last saw in the part, so
+    // they shrink as the part drains.  A wave exits after finding every part
+    // drained.
+    const long long gw = (long long)blockIdx.x * 4 + wv;
+    long long s_cur = gw * A.first_take, s_end = s_cur + A.first_take;
+    if (s_end > A.static_end) s_end = A.static_end;
+    int qx = (int)(blockIdx.x % CSE_NQUEUES), tried = 0;
+    long long q0 = A.static_end + (long long)qx * A.part, q1 = q0 + A.part;
+    if (q1 > A.nitems) q1 = A.nitems;
+    long long seen = q0;
     while (true) {
         long long item = 0;
-        if (x == 0) item = (long long)atomicAdd((unsigned long long *)A.queue, 1ull);
-        item = (long long)rl64((u64)item, 0);
-        if (item >= A.nitems) break;
+        if (mode == 0) {
+            if (x == 0) item = (long long)atomicAdd((unsigned long long *)A.queue, 1ull);
+            item = (long long)rl64((u64)item, 0);
+            if (item >= A.nitems) break;
+        } else {
+            while (s_cur >= s_end && tried < CSE_NQUEUES) {
+                long long base = 0, take = 0;
+                if (x == 0) {
+                    take = (q1 - seen) / A.waves_per_part;
+                    if (take > A.take_max) take = A.take_max;
+                    if (take < 1) take = 1;
+                    base = q0 + (long long)atomicAdd(A.queue + (size_t)qx * CSE_QUEUE_STRIDE,
+                                                     (unsigned long long)take);
+                }
+                base = (long long)rl64((u64)base, 0);
+                take = (long long)rl64((u64)take, 0);
+                if (base < q1) {
+                    s_cur = base;
+                    s_end = base + take < q1 ? base + take : q1;
+                    seen = s_end;
+                } else {
+                    // this part is drained: the next one (every claim moves a
+                    // counter on, so each part drains and every wave leaves)
+                    tried++;
+                    qx = qx + 1 < CSE_NQUEUES ? qx + 1 : 0;
+                    q0 = A.static_end + (long long)qx * A.part;
+                    q1 = q0 + A.part < A.nitems ? q0 + A.part : A.nitems;
+                    seen = q0;
+                }
+            }
+            if (s_cur >= s_end) break;
+            item = s_cur;
+        }
 
         const int inst = A.ninst > 1 ? find_instance(A.item_off, A.ninst, item) : 0;
         const InstDesc &D = A.inst[inst];
         const long long local = item - A.item_off[inst];
-        long long r, c;
-        if (A.mode == 0) {
+        long long r, c, nsl = 1;
+        if (mode == 0) {
             r = local / D.nchunks;
             c = local % D.nchunks;
         } else {
             tri_item(local, D.chunk, D.nchunks, r, c);
+            // one segment: the claimed slices of record r (consecutive in
+            // the enumeration, from slice c to the record's last)
+            nsl = s_end - s_cur;
+            if (nsl > D.nchunks - c) nsl = D.nchunks - c;
+            s_cur += nsl;
         }
         long long j0 = c * (long long)D.chunk;
-        long long j1 = j0 + D.chunk;
+        long long j1 = j0 + nsl * (long long)D.chunk;
         if (j1 > D.chi) j1 = D.chi;
-        if (A.mode == 1 && j0 < r) j0 = r;
+        if (mode == 1 && j0 < r) j0 = r;
         if (j0 >= j1) continue;
+        if (PERSIST && inst != cur) {
+            flush_instance();
+            cur = inst;
+        }
         // first per-term output slot of j0
-        const long long tbase = A.mode == 0 ? (D.term_off + r * D.chi + j0)
+        const long long tbase = mode == 0 ? (D.term_off + r * D.chi + j0)
                                             : (D.term_off + r * D.chi - r * (r - 1) / 2 + (j0 - r));
 
         const uint64_t *rec = A.recs + (size_t)REC_WORDS * (size_t)(D.rec_off + r);
@@ -1235,10 +1324,11 @@ DEV void sweep_body(const SweepArgs &A) {
             continue;
         }
 
-        LaneAcc acc;
-        acc.clear();
+        LaneAcc iacc;
+        iacc.clear();
+        LaneAcc &acc = PERSIST ? pacc : iacc;
         // instance fields used per term, held in registers
-        const int n = D.n, mode = A.mode;
+        const int n = D.n;
         int bad = 0;  // per lane: a term outside p in [-64, 0] (never folded)
         // a batch of up to 64 terms, one parked per lane, then folded by all
         // lanes; in mode 1 every term but j = r counts twice (2 Re)
@@ -1248,7 +1338,7 @@ DEV void sweep_body(const SweepArgs &A) {
                 A.terms[tbase + (jb - j0) + x] = parked ? ((v >> 1) + 1) : 0;
             }
             const int diag = (mode == 1 && r >= jb && r < jb + nb) ? (int)(r - jb) : -1;
-            bad |= fold_terms(parked, nb, mode, diag, rv.sexp, acc, hist);
+            bad |= fold_terms(parked, nb, mode, diag, rv.sexp, acc, hist, PERSIST ? rv.sexp * 8 : 0);
         };
         if (D.exact && PK) {
             // the whole batch in one asm block
@@ -1366,11 +1456,13 @@ DEV void sweep_body(const SweepArgs &A) {
             }
         }
         if (ballot(bad) && x == 0) atomicOr(A.err, 2);
+        if (x == 0) atomicAdd((unsigned long long *)&A.ips[inst], (unsigned long long)(j1 - j0));
+        if (PERSIST) continue;
         // flush the item (the LDS histogram atomics of this wave come first)
         __threadfence_block();
         const double kr = wave_sumf64((double)acc.sr - (double)acc.cr);
         const double ki = wave_sumf64((double)acc.si - (double)acc.ci);
-        if (A.mode == 0) {
+        if (mode == 0) {
             const long long z0 = wave_sum64(acc.c0), z1 = wave_sum64(acc.c1);
             const long long z2 = wave_sum64(acc.c2), z3 = wave_sum64(acc.c3);
             if (x == 0) {
@@ -1389,7 +1481,7 @@ DEV void sweep_body(const SweepArgs &A) {
             atomicAdd(&kz[1], ki);
         }
         const int shift = rv.sexp * 8;
-        for (int b = x; A.mode == 1 && b < LH_BINS; b += 64) {
+        for (int b = x; mode == 1 && b < LH_BINS; b += 64) {
             unsigned int v = hist[b];
             if (v) {
                 atomicAdd((unsigned long long *)&A.hist[(size_t)inst * A.hist_stride + b + shift],
@@ -1397,25 +1489,44 @@ DEV void sweep_body(const SweepArgs &A) {
                 hist[b] = 0;
             }
         }
-        if (x == 0) atomicAdd((unsigned long long *)&A.ips[inst], (unsigned long long)(j1 - j0));
     }
+    if (PERSIST) flush_instance();
 }
 
-extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_body<false, false>(A); }
+extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_body<false, false, -1>(A); }
 
 // the same sweep with D packed into the J rows (every instance has n <= 60;
 // row 60 is the elimination loop's end row).
 // Eight waves per SIMD (63 VGPRs, 78 SGPRs, one 8-byte spill stored at entry
 // and reloaded at exit): the elimination loop is bound by instruction issue,
 // and the extra wave hides more of its VALU -> SALU dependency latency.
-extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed(SweepArgs A) { sweep_body<true, false>(A); }
+extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed(SweepArgs A) {
+    sweep_body<true, false, 0>(A);
+}
+
+// The sampled sweep with the general restriction loop of exact mode: every
+// |L> term runs in the asm block whatever the record's number of non-pivot
+// columns, every |H^t> term in phiH_batch with the batch of uncoupled
+// variables.  Eight waves per SIMD (62 VGPRs).  A tuning alternative to
+// k_sweep_packed (cse_set_sweep_kernel); the results are identical.
+extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed_gen(SweepArgs A) {
+    sweep_body<true, true, 0>(A);
+}
 
 // The packed sweep of exact mode: exact-mode records (projected |L> states of dimension about |x_i|) have
 // about n/2 non-pivot columns, so every |L> term runs in the asm block with
 // its general restriction loop, and every |H^t> term in phiH_batch.  The working G
 // rows take two more VGPRs: seven waves per SIMD, without spills.
-extern "C" __global__ void __launch_bounds__(256, 7) k_sweep_packed_all(SweepArgs A) {
-    sweep_body<true, true>(A);
+extern "C" __global__ void __launch_bounds__(256, CSE_EXACT_BLOCKS_PER_CU) k_sweep_packed_all(SweepArgs A) {
+    sweep_body<true, true, 1>(A);
+}
+
+// The same exact-mode sweep at eight waves per SIMD (64 VGPRs, more SGPRs
+// spilled to VGPR lanes): more latency hidden, more spill traffic.  A tuning
+// alternative (cse_set_sweep_kernel); large mixed launches gain a few
+// percent, launches of many small instances lose a few.
+extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed_all8(SweepArgs A) {
+    sweep_body<true, true, 1>(A);
 }
 
 // Histogram of the Hamming weight |x| over the 2^k combinations

@@ -12,11 +12,17 @@
 
 #include "cse.h"
 
+/* The context replays its batched calls from captured HIP graphs
+ * (cse_set_graphs): repeated probability evaluations and sampling rounds
+ * have the same shape call after call.  CSE_GRAPHS=0 in the environment
+ * keeps them eager. */
 static int ensure_ctx(cse_ctx **ctx, int device)
 {
     if (*ctx) return CSE_OK;
     int rc = cse_create(device, ctx);
-    return rc == CSE_OK ? CSE_OK : CSE_ERR_NODEV;
+    if (rc != CSE_OK) return CSE_ERR_NODEV;
+    const char *g = getenv("CSE_GRAPHS");
+    return cse_set_graphs(*ctx, !(g && strcmp(g, "0") == 0));
 }
 
 /* 1 + #{+I} - #{-I} over the generators (innerprod.c:157-167) */

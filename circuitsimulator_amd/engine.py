@@ -26,6 +26,7 @@ ABI_SYMBOLS = [
     "cse_prep_states", "cse_random_states", "cse_exact_projector", "cse_sampled_projector",
     "cse_batch_exact", "cse_decompose", "cse_hist_value", "cse_sampling_kept", "cse_projector_norm", "cse_projector_norms",
     "cse_lspace_weights", "cse_batch_sampled", "cse_decompose_many", "cse_exact_projector_noalloc",
+    "cse_set_graphs", "cse_set_sweep_kernel",
 ]
 ERR_NODEV = -5
 
@@ -104,6 +105,8 @@ def lib():
         L.cse_error.argtypes = [ctypes.c_void_p]
         L.cse_error.restype = ctypes.c_char_p
         L.cse_device_info.argtypes = [ctypes.c_void_p, P(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]
+        L.cse_set_graphs.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cse_set_sweep_kernel.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.cse_inner_products.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, P(U64), P(U64),
                                          P(ctypes.c_int32), ctypes.c_int, P(ctypes.c_double)]
         L.cse_prep_states.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(U64), ctypes.c_int,
@@ -152,6 +155,7 @@ class Engine:
 
     def __init__(self, device=0):
         self._L = lib()
+        self.sweep_kernels = [0, 0]
         h = ctypes.c_void_p()
         rc = self._L.cse_create(device, ctypes.byref(h))
         if rc != 0:
@@ -172,6 +176,23 @@ class Engine:
     def _check(self, rc):
         if rc != 0:
             raise EngineError(f"engine error {rc}: {self._L.cse_error(self.h).decode()}")
+
+    def set_graphs(self, enable=True):
+        """replay batched calls of a repeated shape from captured HIP graphs
+        (cse_set_graphs)"""
+        self._check(self._L.cse_set_graphs(self.h, 1 if enable else 0))
+
+    def set_sweep_kernel(self, kernel, mode=0):
+        """sweep kernel of one mode (cse_set_sweep_kernel): sampled (mode 0):
+        0 the engine's choice, 1 the eight-wave kernel with the two-column
+        asm path, 2 the general one; exact (mode 1): 0 the engine's choice,
+        1 seven waves per SIMD, 2 eight"""
+        self._check(self._L.cse_set_sweep_kernel(self.h, int(mode), int(kernel)))
+        self.sweep_kernels[int(mode)] = int(kernel)
+
+    @property
+    def sweep_kernel(self):
+        return self.sweep_kernels[0]
 
     def device_info(self):
         cus = ctypes.c_int()
@@ -262,13 +283,15 @@ class Engine:
         return hist
 
     def batch_exact(self, instances, chunk=0, want_terms=False, nterms=0):
-        """instances: a list of Instance or an instance_array.  chunk: the item
-        size in j (0: the engine's default; "auto": the autotuner's choice)."""
+        """instances: a list of Instance or an instance_array.  chunk: the
+        claim cap in j (0: the engine's default; "auto": the autotuner's
+        choice of kernel and cap for this call)."""
+        kernel = None
         if isinstance(chunk, str):
             if chunk != "auto":
                 raise ValueError(f"chunk must be an int or 'auto', not {chunk!r}")
-            from .autotune import tune_exact_chunk
-            chunk = tune_exact_chunk(self, instances)
+            from .autotune import tune_exact
+            kernel, chunk = tune_exact(self, instances)
         if isinstance(instances, np.ndarray):
             if instances.dtype != INSTANCE_DTYPE or not instances.flags["C_CONTIGUOUS"]:
                 raise ValueError("instances must be a contiguous instance_array")
@@ -278,9 +301,16 @@ class Engine:
         res = (ExactResult * len(instances))()
         terms = np.zeros(nterms, dtype=np.int32) if want_terms else None
         ms = ctypes.c_double()
-        self._check(self._L.cse_batch_exact(self.h, arr, len(instances), chunk, res,
-                                            _ptr(terms, ctypes.c_int32) if want_terms else None,
-                                            ctypes.byref(ms)))
+        keep = self.sweep_kernels[1]
+        if kernel is not None and kernel != keep:
+            self.set_sweep_kernel(kernel, mode=1)
+        try:
+            self._check(self._L.cse_batch_exact(self.h, arr, len(instances), chunk, res,
+                                                _ptr(terms, ctypes.c_int32) if want_terms else None,
+                                                ctypes.byref(ms)))
+        finally:
+            if kernel is not None and kernel != keep:
+                self.set_sweep_kernel(keep, mode=1)
         return list(res), terms, ms.value
 
     def batch_sampled(self, instances, chunk=0, want_samples=False):
@@ -288,14 +318,15 @@ class Engine:
         kernel ms, samples); means[i] is cse_sampled_projector's mean for
         instance i; samples[i] (want_samples) its per-sample zsum and sexp.
         chunk: the item size in j (0: the engine's default; "auto": the
-        autotuner's choice)."""
+        autotuner's choice of kernel and item size for this call)."""
         if instances.dtype != SAMPLED_INSTANCE_DTYPE or not instances.flags["C_CONTIGUOUS"]:
             raise ValueError("instances must be a contiguous sampled_instance_array")
+        kernel = None
         if isinstance(chunk, str):
             if chunk != "auto":
                 raise ValueError(f"chunk must be an int or 'auto', not {chunk!r}")
-            from .autotune import tune_sampled_chunk
-            chunk = tune_sampled_chunk(self, instances)
+            from .autotune import tune_sampled
+            kernel, chunk = tune_sampled(self, instances)
         n = len(instances)
         res = (SampledResult * n)()
         samples = []
@@ -307,8 +338,15 @@ class Engine:
                 res[i].sexp = _ptr(extra["sexp"], ctypes.c_int32)
                 samples.append(extra)
         ms = ctypes.c_double()
-        self._check(self._L.cse_batch_sampled(self.h, instances.ctypes.data_as(P(SampledInstance)), n, chunk,
-                                              res, ctypes.byref(ms)))
+        keep = self.sweep_kernel
+        if kernel is not None and kernel != keep:
+            self.set_sweep_kernel(kernel)
+        try:
+            self._check(self._L.cse_batch_sampled(self.h, instances.ctypes.data_as(P(SampledInstance)), n, chunk,
+                                                  res, ctypes.byref(ms)))
+        finally:
+            if kernel is not None and kernel != keep:
+                self.set_sweep_kernel(keep)
         return np.array([r.mean for r in res]), list(res), ms.value, samples
 
 
@@ -325,6 +363,9 @@ def shared_engine(device=0):
     e = _shared.get(key)
     if e is None or not e.h:
         e = _shared[key] = Engine(device)
+        # its batches repeat their shape round after round: replay them from
+        # captured graphs unless CSE_GRAPHS=0
+        e.set_graphs(os.environ.get("CSE_GRAPHS", "1") != "0")
     return e
 
 

@@ -58,6 +58,32 @@ const char *cse_error(const cse_ctx *ctx);
 /* number of compute units and the device name, for reports */
 int cse_device_info(cse_ctx *ctx, int *cus, char *name, int name_len);
 
+/*
+ * enable != 0: the context's batched calls (cse_exact_projector,
+ * cse_sampled_projector, cse_batch_exact, cse_batch_sampled and the
+ * projector norms) replay HIP graphs: the first call of a given shape
+ * (instance count, records, work items, buffers) captures its plan copy,
+ * clearing launch, record launch, sweep and results copy, and later calls
+ * of that shape launch the captured graph after writing their plan into
+ * the same pinned stage.  Results are identical; calls that return
+ * per-term output always run eagerly.  No reference counterpart: the
+ * reference starts one backend process per probability() call
+ * (probability.py:237-244).  Default off.
+ */
+int cse_set_graphs(cse_ctx *ctx, int enable);
+
+/*
+ * The sweep kernel of the context's calls of one mode, a tuning knob (the
+ * results are identical).  mode 0 (sampled calls): kernel 0 the engine's
+ * choice (1), 1 the eight-wave sweep whose hand-scheduled path takes
+ * records of at most two non-pivot columns (others run compiled code), 2
+ * the eight-wave sweep whose hand-scheduled path takes any number of
+ * non-pivot columns and, for |H^t> terms, retires uncoupled variables in
+ * one batch.  mode 1 (exact calls): kernel 0 the engine's choice (1), 1 the
+ * exact sweep at seven waves per SIMD, 2 the same at eight.  No reference
+ * counterpart (the reference has one CPU code path).
+ */
+int cse_set_sweep_kernel(cse_ctx *ctx, int mode, int kernel);
 
 /*
  * <s2[i] | s1[i]> for npairs state pairs of dimension n: one wavefront per
@@ -127,7 +153,7 @@ int cse_exact_projector(cse_ctx *ctx, const cse_projector *P, const uint64_t *L,
  * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls
  * no hipMalloc, hipFree, host allocator or other allocating routine: every
  * device buffer the call needs (work queue, plan, results block, records)
- * lives in a static device array of 8,761,856 bytes that each GPU holds
+ * lives in a static device array of 8,762,624 bytes that each GPU holds
  * until the process ends, and the host stages are on the stack.  Results
  * equal cse_exact_projector's.  chi <= 2^13 (|H^t> with t <= 26, or k <= 13);
  * larger chi returns CSE_ERR_ARG.  Every call on one GPU shares that GPU's

@@ -1,6 +1,7 @@
-"""The item-size autotuner's host logic: item counts against enumeration of
-the sweep's work items, the proxy it times and its cache (a stand-in engine
-that records what it is asked to run; no GPU)."""
+"""The autotuner's host logic: item counts against enumeration of the
+sweep's work items, the candidates it times (claim caps in exact mode;
+kernel and item size in sampled mode), the proxy and its cache (a stand-in
+engine that records what it is asked to run; no GPU)."""
 import numpy as np
 import pytest
 
@@ -32,23 +33,29 @@ def test_exact_items_at_the_two_point_workload():
 
 class FakeEngine:
     """Answers device_info and records each batch it is asked to time; the
-    sweep time of a candidate is `cost[chunk]`."""
+    sweep time of a candidate is `cost[(kernel, chunk)]`."""
 
     def __init__(self, cost, cus=4):
         self.cost = cost
         self.cus = cus
         self.runs = []
+        self.sweep_kernels = [0, 0]
 
     def device_info(self):
         return self.cus, "fake"
 
+    def set_sweep_kernel(self, kernel, mode=0):
+        self.sweep_kernels[mode] = kernel
+
     def batch_exact(self, instances, chunk=0):
-        self.runs.append(("exact", chunk, instances))
-        return [], None, self.cost[chunk]
+        kc = (self.sweep_kernels[1], chunk)
+        self.runs.append(("exact", kc, instances))
+        return [], None, self.cost[kc]
 
     def batch_sampled(self, instances, chunk=0):
-        self.runs.append(("sampled", chunk, instances))
-        return None, [], self.cost[chunk], []
+        kc = (self.sweep_kernels[0], chunk)
+        self.runs.append(("sampled", kc, instances))
+        return None, [], self.cost[kc], []
 
 
 def exact_array(ks, t=20):
@@ -60,25 +67,44 @@ def exact_array(ks, t=20):
     return A
 
 
-COST = {0: 2.0, 128: 3.0, 192: 2.6, 256: 2.0, 384: 1.8, 512: 1.5, 1024: 2.5}
+def cost_table(sizes, best, base=2.0, slow=None):
+    """sweep times of every (kernel, size) and the rule (0, 0): the kernel-2
+    candidates 10% slower, the size adding a little, `best` at half, and
+    `slow` entries overriding"""
+    cost = {(0, 0): base}
+    for kern in autotune.KERNELS:
+        for c in (0,) + tuple(sizes):
+            cost[(kern, c)] = base * (1.1 if kern == 2 else 1.0) + 0.001 * c / 64
+    if best is not None:
+        cost[best] = 0.5 * base
+    cost.update(slow or {})
+    return cost
+
+
+def exact_cost(best=(1, 512), **kw):
+    return cost_table(autotune.EXACT_CAPS, best, **kw)
 
 
 def test_exact_tuner_picks_the_fastest_and_caches():
     autotune.clear_cache()
-    E = FakeEngine(COST)
+    cost = exact_cost(best=(2, 1024))
+    E = FakeEngine(cost)
     A = exact_array([2, 3, 4, 5])
     timings = {}
-    assert autotune.tune_exact_chunk(E, A, timings=timings) == 512
-    assert timings == COST
-    # two repetitions per candidate, on the whole (small) workload
-    assert [r[1] for r in E.runs] == [0, 0, 128, 128, 192, 192, 256, 256, 384, 384, 512, 512, 1024, 1024]
+    assert autotune.tune_exact(E, A, timings=timings) == (2, 1024)
+    assert timings == cost
+    # two repetitions per candidate, on the whole (small) workload, the
+    # engine's rule first; the context's kernel is left as it was
+    kcs = [(0, 0)] + [(k, c) for k in autotune.KERNELS for c in (0,) + autotune.EXACT_CAPS]
+    assert [r[1] for r in E.runs] == [kc for kc in kcs for _ in range(2)]
     assert all(len(r[2]) == 4 for r in E.runs)
+    assert E.sweep_kernels == [0, 0]
     n = len(E.runs)
-    assert autotune.tune_exact_chunk(E, A) == 512
+    assert autotune.tune_exact(E, A) == (2, 1024)
     assert len(E.runs) == n
     # a list of Instance takes the same key
     lst = [autotune_instance(k) for k in [2, 3, 4, 5]]
-    assert autotune.tune_exact_chunk(E, lst) == 512
+    assert autotune.tune_exact(E, lst) == (2, 1024)
     assert len(E.runs) == n
 
 
@@ -92,10 +118,10 @@ def autotune_instance(k, t=20):
 
 def test_exact_proxy_keeps_the_largest_instances():
     autotune.clear_cache()
-    E = FakeEngine(COST, cus=1)
+    E = FakeEngine(exact_cost(), cus=1)
     ks = [2] * 50 + [12] + [3] * 50 + [11]
     A = exact_array(ks, t=30)
-    autotune.tune_exact_chunk(E, A, items_per_wave=2)
+    autotune.tune_exact(E, A, items_per_wave=2)
     proxy = E.runs[0][2]
     # target 2 items x 28 waves; the k=12 instance alone has more
     assert 0 < len(proxy) < len(A)
@@ -107,22 +133,26 @@ def test_exact_proxy_keeps_the_largest_instances():
 
 def test_exact_key_separates_shapes():
     autotune.clear_cache()
-    E = FakeEngine(COST)
-    autotune.tune_exact_chunk(E, exact_array([4] * 3, t=20))
+    E = FakeEngine(exact_cost())
+    autotune.tune_exact(E, exact_array([4] * 3, t=20))
     n = len(E.runs)
     # wider than the packed layout: another key
-    autotune.tune_exact_chunk(E, exact_array([4] * 3, t=62))
+    autotune.tune_exact(E, exact_array([4] * 3, t=62))
     assert len(E.runs) == 2 * n
     # the |H^t> decomposition: another key
     A = exact_array([0] * 3, t=8)
     A["exact"] = 1
-    autotune.tune_exact_chunk(E, A)
+    autotune.tune_exact(E, A)
     assert len(E.runs) == 3 * n
+
+
+def sampled_cost(best=(2, 384), **kw):
+    return cost_table(autotune.CANDIDATES, best, **kw)
 
 
 def test_sampled_proxy_scales_counts():
     autotune.clear_cache()
-    E = FakeEngine(COST, cus=1)
+    E = FakeEngine(sampled_cost(best=(1, 512)), cus=1)
     A = sampled_instance_array(3)
     A["P"]["nqubits"] = 24
     A["P"]["nstabs"] = 1
@@ -131,7 +161,7 @@ def test_sampled_proxy_scales_counts():
     A["first"] = [0, 100, 200]
     A["count"] = [4096, 1024, 3]
     timings = {}
-    assert autotune.tune_sampled_chunk(E, A, items_per_wave=4, timings=timings) == 512
+    assert autotune.tune_sampled(E, A, items_per_wave=4, timings=timings) == (1, 512)
     proxy = E.runs[0][2]
     assert proxy.flags["C_CONTIGUOUS"]
     assert list(proxy["seed"]) == [5, 6, 7] and list(proxy["first"]) == [0, 100, 200]
@@ -144,16 +174,35 @@ def test_sampled_proxy_scales_counts():
     assert list(A["count"]) == [4096, 1024, 3]
 
 
+def test_sampled_tuner_times_every_kernel_and_size_and_restores_the_kernel():
+    autotune.clear_cache()
+    E = FakeEngine(sampled_cost(best=(2, 384)))
+    E.sweep_kernels[0] = 1
+    A = sampled_instance_array(2)
+    A["P"]["nqubits"] = 20
+    A["P"]["nstabs"] = 1
+    A["k"] = [6, 7]
+    A["count"] = [10, 20]
+    timings = {}
+    assert autotune.tune_sampled(E, A, timings=timings) == (2, 384)
+    assert sorted(timings) == sorted([(0, 0)] + [(k, c) for k in autotune.KERNELS
+                                                 for c in (0,) + autotune.CANDIDATES])
+    assert E.sweep_kernels[0] == 1
+    # a candidate within the tolerance of the engine's rule leaves the rule
+    autotune.clear_cache()
+    assert autotune.tune_sampled(FakeEngine(sampled_cost(best=None, slow={(2, 384): 1.97})), A) == (0, 0)
+
+
 def test_the_engine_rule_stays_within_the_tolerance():
-    # a fixed size that wins by less than tol is noise: 0 (the engine's rule)
+    # a candidate that wins by less than tol is noise: (0, 0), the engine's rule
     autotune.clear_cache()
-    E = FakeEngine({0: 2.0, 128: 2.1, 192: 2.05, 256: 2.0, 384: 1.96, 512: 1.95, 1024: 2.2})
-    assert autotune.tune_exact_chunk(E, exact_array([3, 4])) == 0
+    cost = exact_cost(best=None, slow={(1, 512): 1.97})
+    assert autotune.tune_exact(FakeEngine(cost), exact_array([3, 4])) == (0, 0)
     autotune.clear_cache()
-    assert autotune.tune_exact_chunk(E, exact_array([3, 4]), tol=0.0) == 512
+    assert autotune.tune_exact(FakeEngine(cost), exact_array([3, 4]), tol=0.0) == (1, 512)
     autotune.clear_cache()
-    E = FakeEngine({0: 2.0, 128: 1.7, 192: 1.8, 256: 2.0, 384: 1.9, 512: 1.9, 1024: 2.2})
-    assert autotune.tune_exact_chunk(E, exact_array([3, 4])) == 128
+    cost = exact_cost(best=None, slow={(2, 256): 1.7, (1, 512): 1.8})
+    assert autotune.tune_exact(FakeEngine(cost), exact_array([3, 4])) == (2, 256)
 
 
 def test_engine_rejects_unknown_chunk_words():
