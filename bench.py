@@ -226,15 +226,17 @@ def chunk_arg(args, default):
 
 
 def tuned_exact_chunk(E, instances, arg):
-    """(item size, tuning record): the autotuner runs before the warmup, outside
-    the timed region, on a proxy of the same instances"""
+    """(claim cap, tuning record): the autotuner runs before the warmup,
+    outside the timed region, on a proxy of the same instances, and leaves
+    its kernel choice set on the context"""
     if arg != "auto":
         return arg, {"size": arg}
-    from circuitsimulator_amd.autotune import tune_exact_chunk
+    from circuitsimulator_amd.autotune import tune_exact
     tm = {}
     t0 = time.perf_counter()
-    c = tune_exact_chunk(E, instances, timings=tm)
-    return c, {"size": c, "tune_s": time.perf_counter() - t0,
+    kern, c = tune_exact(E, instances, timings=tm)
+    E.set_sweep_kernel(kern, mode=1)
+    return c, {"kernel": kern, "size": c, "tune_s": time.perf_counter() - t0,
                "proxy_sweep_ms": {str(key): round(v, 4) for key, v in tm.items()}}
 
 
@@ -257,6 +259,8 @@ def exact_pairs_config(args):
     for i, v in enumerate(L):
         inst.L[i] = v
     with Engine(0) as E:
+        if args.kernel:
+            E.set_sweep_kernel(args.kernel, mode=1)
         chunk, tuning = tuned_exact_chunk(E, [inst], chunk_arg(args, "auto"))
         for _ in range(args.warmup):
             E.batch_exact([inst], chunk=chunk)
@@ -293,6 +297,8 @@ def sampled_h_config(args):
     P, _, _ = sampled_config(t, r, 1, seed=1040)
     chi = 1 << ((t + 1) // 2)
     with Engine(0) as E:
+        if args.kernel:
+            E.set_sweep_kernel(args.kernel, mode=0)
         for w in range(args.warmup):
             E.sampled_projector(P, [], 0, 1, 7, 0, 64, want_samples=False)
         t0 = time.perf_counter()
@@ -327,6 +333,8 @@ def exact_h_config(args):
     chi = 1 << ((t + 1) // 2)
     nterms = chi * (chi + 1) // 2
     with Engine(0) as E:
+        if args.kernel:
+            E.set_sweep_kernel(args.kernel, mode=1)
         for _ in range(args.warmup):
             E.exact_projector(P, [], 0, 1)
         t0 = time.perf_counter()
@@ -357,6 +365,8 @@ def batch_config(args):
     inst, specs = uneven_instances(count, seed=1003)
     nterms = sum((1 << k) * ((1 << k) + 1) // 2 for (_, _, k, *_rest) in specs)
     with Engine(0) as E:
+        if args.kernel:
+            E.set_sweep_kernel(args.kernel, mode=1)
         chunk, tuning = tuned_exact_chunk(E, inst, chunk_arg(args, "auto"))
         for _ in range(args.warmup):
             E.batch_exact(inst, chunk=chunk)
@@ -407,6 +417,94 @@ def circuit_config(args):
     }
 
 
+def exact_projector_oracle(rec):
+    """exactProjector (innerprod.c:148-199) of one cse_instance record by the
+    oracle: |sum_{i<=j} <phi_j|Pi phi_i>| (2 Re off the diagonal), summed
+    exactly in Q(sqrt2) and rounded once to FP64."""
+    import ctypes
+    from decimal import Decimal, getcontext
+    from fractions import Fraction
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    t, g = int(rec["P"]["nqubits"]), int(rec["P"]["nstabs"])
+    ex, k = int(rec["exact"]), int(rec["k"])
+    Po = orc.make_projector([int(v) for v in rec["P"]["phase"][:g]], [int(v) for v in rec["P"]["xs"][:g]],
+                            [int(v) for v in rec["P"]["zs"][:g]], t)
+    Larr = (orc.U64 * max(1, k))(*([int(v) for v in rec["L"][:k]] if k else [0]))
+    chi = (1 << ((t + 1) // 2)) if ex else (1 << k)
+    # real part: sum of c 2^{P/2} cos(pi M/4) = A + B sqrt2 (exact-mode totals are real)
+    A, B = Fraction(0), Fraction(0)
+    i_, j_, ip, se = orc.U64(), orc.U64(), orc.IpRes(), ctypes.c_int()
+    for l in range(chi * (chi + 1) // 2):
+        if orc.lib().orc_exact_term(l, ctypes.byref(Po), Larr, k, ex, ctypes.byref(i_), ctypes.byref(j_),
+                                    ctypes.byref(ip), ctypes.byref(se)) or not ip.eps:
+            continue
+        P, M = ip.p - se.value, ip.m % 8
+        w = 1 if i_.value == j_.value else 2      # 2 Re(z) off the diagonal
+        cos2 = {0: (1, 0), 1: (0, Fraction(1, 2)), 2: (0, 0), 3: (0, Fraction(-1, 2)), 4: (-1, 0),
+                5: (0, Fraction(-1, 2)), 6: (0, 0), 7: (0, Fraction(1, 2))}[M]  # cos = a + b sqrt2
+        scale = Fraction(2) ** (P // 2)
+        a, b = cos2
+        if P % 2:   # times sqrt2
+            a, b = 2 * b, a
+        A += w * scale * a
+        B += w * scale * b
+    getcontext().prec = 60
+    v = Decimal(A.numerator) / Decimal(A.denominator) + Decimal(B.numerator) / Decimal(B.denominator) * \
+        Decimal(2).sqrt()
+    return float(abs(v))
+
+
+def c5_error_leg(circ, nq, shots=8, k=5, seed=1006):
+    """Per-shot |error| of the sampling workload: an untimed run of the same
+    circuit at k = 5 (chi = 32: the oracle's exact pair sums take seconds)
+    for a seeded subsample of shots, every engine value checked against the
+    oracle's exact projector sum (rounded once), and each shot's
+    probabilities and FP64 conditional-probability ratios recomputed from
+    the oracle's values along the same draws (sample.py:47,70)."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+
+    from circuitsimulator_amd.sample import sampleShots
+    trace = []
+    sampleShots(circ, {}, list(range(nq)), shots,
+                config={"exact": False, "k": k, "quiet": True, "rng": np.random.default_rng(seed), "trace": trace})
+    batches = [e for e in trace if "instances" in e]
+    recs = [rec for b in batches for rec in b["instances"]]
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as pool:
+        ovals = list(pool.map(exact_projector_oracle, recs))
+    engine_vals = [v for b in batches for v in b["values"]]
+    val_err = max((abs(v - o) / max(abs(o), 1e-300) for v, o in zip(engine_vals, ovals)), default=0.0)
+    # P per round from the oracle's values, then the chain of ratios
+    it = iter(ovals)
+    perr = np.zeros(shots)
+    p0err = np.zeros(shots)
+    pending = None
+    psofar = np.ones(shots)
+    for e in trace:
+        if "instances" in e:
+            ov = np.array([next(it) for _ in range(len(e["instances"]))])
+            Por = e["P"].copy()
+            todo, inv, nnum = e["todo"], e["inv"], e["nnum"]
+            num = ov[:nnum][inv]
+            den = np.full(len(todo), ov[nnum]) if e["exact"] and len(ov) > nnum else (
+                ov[nnum:] if len(ov) > nnum else np.full(len(todo), e["norm"] ** 2))
+            Por[todo] = np.where(num == 0, 0.0, e["base"] * num / np.where(num == 0, 1.0, den))
+            pending = Por
+            continue
+        Por = pending if pending is not None else e["P"]
+        pending = None
+        P0or = Por / psofar
+        perr = np.maximum(perr, np.abs(e["P"] - Por))
+        p0err = np.maximum(p0err, np.abs(e["P0"] - P0or))
+        psofar = np.where(e["draw"] == 0, P0or, 1 - P0or)
+    return {"shots": shots, "k": k, "seed": seed, "instances_checked": len(recs),
+            "max_rel_err_projector_values": val_err,
+            "per_shot_abs_err_P": [float(x) for x in perr],
+            "per_shot_abs_err_P0": [float(x) for x in p0err],
+            "max_abs_err_P0": float(p0err.max())}
+
+
 def shots_config(args):
     """C5: qubit-by-qubit sampling of 8 qubits of a seeded 40-qubit circuit
     of 400 {H, S, CX} gates and 40 T gates (t = 40), 1024 shots, |L> with
@@ -428,6 +526,7 @@ def shots_config(args):
     freq = {}
     for o in out:
         freq[o] = freq.get(o, 0) + 1
+    check = None if args.no_cpu_baseline else c5_error_leg(circ, 8)
     return {
         "metric": "sampled shots/s, 8 qubits of a 40-qubit Clifford+T circuit (t=40)",
         "value": shots / dt, "unit": "shots/s", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
@@ -437,6 +536,7 @@ def shots_config(args):
                                f"{shots} shots", "seed": 1005, "shots": shots, "k": 8},
         "inner_products": shots * 8 * 2 * (256 * 257 // 2),
         "distinct_outcomes": len(freq),
+        "error_vs_fp64_oracle": check,
     }
 
 
@@ -448,8 +548,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1; c1: 20)")
     ap.add_argument("--samples", type=int, default=65536, help="random thetas per step (M)")
     ap.add_argument("--chunk", default=None,
-                    help="item size in j, 0 for the engine's rule, or auto for the autotuner's choice "
-                         "(default: auto for c2 and c3, 0 otherwise)")
+                    help="item size in j (exact mode: the claim cap), 0 for the engine's rule, or auto for the "
+                         "autotuner's choice (c4: kernel and item size) (default: auto for c2 and c3, 0 otherwise)")
     ap.add_argument("--plain-pairs", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5", "h30", "hs40"], default="c4",
@@ -463,6 +563,8 @@ def main():
     ap.add_argument("--ref-opt", choices=["O2", "O0"], default="O2",
                     help="optimisation level of the reference C build the CPU baseline times")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: cores, at most 16)")
+    ap.add_argument("--kernel", type=int, default=0, choices=[0, 1, 2],
+                    help="sweep kernel of the configuration's mode (cse_set_sweep_kernel; 0: the engine's choice)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 200 if args.config == "c1" else 3
@@ -501,12 +603,14 @@ def main():
         # non-vanishing draws of timed step 0 (seed 1004)
         _, _, _, ncheck = baseline_thetas(P_tuple, t, 1004, cpu_threads)
     with Engine(0) as E:
+        if args.kernel:
+            E.set_sweep_kernel(args.kernel, mode=0)
         cus, devname = E.device_info()
         chunk = chunk_arg(args, 0)
         tuning = {"size": chunk}
         if chunk == "auto":
             # the same draw as one step, as a one-instance sampled batch
-            from circuitsimulator_amd.autotune import tune_sampled_chunk
+            from circuitsimulator_amd.autotune import tune_sampled
             from circuitsimulator_amd.engine import sampled_instance_array
             A = sampled_instance_array(1)
             A[0]["P"] = np.frombuffer(bytes(P), dtype=A.dtype["P"])[0]
@@ -515,8 +619,9 @@ def main():
             A[0]["seed"], A[0]["count"] = 1004, M
             tm = {}
             t1 = time.perf_counter()
-            chunk = tune_sampled_chunk(E, A, timings=tm)
-            tuning = {"size": chunk, "tune_s": time.perf_counter() - t1,
+            kern, chunk = tune_sampled(E, A, timings=tm)
+            E.set_sweep_kernel(kern)
+            tuning = {"kernel": kern, "size": chunk, "tune_s": time.perf_counter() - t1,
                       "proxy_sweep_ms": {str(key): round(v, 4) for key, v in tm.items()}}
         for w in range(args.warmup):
             E.sampled_projector(P, L, k, 0, 1004 + 100 + w, 0, M, chunk=chunk, want_samples=False)

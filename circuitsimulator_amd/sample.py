@@ -254,6 +254,10 @@ def sampleShots(circ, measure, sample, shots, config={}):
           [sum(int(b) << j for j, b in enumerate(z)) for z in Hp[2]])
     device = int(cfg.get("device") or 0)
     engine = None
+    # config["trace"] (a list): every exact batch of the run is appended to
+    # it (its instance array, the engine's values, how they combine into
+    # each shot's P), then each round's P, P0 and draws, for checks
+    trace = cfg.get("trace")
 
     def fill(A, rows, phases, xs, zs, ex, k, Ls):
         """Records rows of A: projector (phases: one row per record or one
@@ -341,6 +345,9 @@ def sampleShots(circ, measure, sample, shots, config={}):
         else:
             den = vals[nnum:]
         out[todo] = np.where(num == 0, 0.0, base * num / np.where(num == 0, 1.0, den))
+        if trace is not None:
+            trace.append({"instances": A.copy(), "values": vals, "todo": todo, "inv": inv, "nnum": nnum,
+                          "base": base, "exact": bool(ex), "norm": norm, "P": out.copy()})
         return out
 
     def round_projectors(m, bits):
@@ -374,5 +381,8 @@ def sampleShots(circ, measure, sample, shots, config={}):
         u01 = rng.random(shots) if rng is not None else np.random.random(shots)
         draw = np.where(u01 < P0, 0, 1)
         bits[:, m - 1] = draw
+        if trace is not None:
+            trace.append({"round": r, "P": P.copy(), "Psofar": np.array(Psofar, dtype=float), "P0": P0.copy(),
+                          "draw": draw.copy()})
         Psofar = _next_psofar(cfg.get("chain", "reference"), P, P0, Psofar, draw)
     return ["".join(str(int(b)) for b in row[ncon:]) for row in bits]
