@@ -518,6 +518,9 @@ def shots_config(args):
     circ, _ = c1_circuit(seed=1005, n=40, gates=400, ts=40)
     shots = args.shots
     cfg = {"exact": False, "k": 8, "quiet": True}
+    if args.kernel:
+        from circuitsimulator_amd.engine import shared_engine
+        shared_engine(0).set_sweep_kernel(args.kernel, mode=1)
     if args.warmup:
         sampleShots(circ, {}, list(range(8)), 8, config=dict(cfg, rng=np.random.default_rng(7)))
     t0 = time.perf_counter()

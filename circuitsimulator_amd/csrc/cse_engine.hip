@@ -89,6 +89,10 @@ struct cse_ctx {
     bool thr_ready = false;
     int blocks_per_cu = 8;  // persistent grid: blocks of 4 waves per CU
     int sweep_kernel[2] = {0, 0};  // per mode (cse_set_sweep_kernel)
+    // exact mode: terms of one instance after which a wave flushes its
+    // partial histogram (CSE_FLUSH_TERMS in the environment, at most and by
+    // default CSE_FLUSH_TERMS_MAX)
+    long long flush_terms = CSE_FLUSH_TERMS_MAX;
 };
 
 // ------------------------------------------------- static scratch array
@@ -501,6 +505,7 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     const bool all8 = all && ctx->sweep_kernel[1] == 2;
     const int grid = sweep_grid(ctx, pl.nitems, all8 ? 8 : all ? CSE_EXACT_BLOCKS_PER_CU : ctx->blocks_per_cu);
     A.take_max = pl.take_max;
+    A.flush_terms = ctx->flush_terms;
     // the static first claims cover about half of the slices (at most
     // take_max per wave); the rest is cut into CSE_NQUEUES parts
     const long long waves = 4ll * grid;
@@ -663,6 +668,12 @@ int init_ctx(cse_ctx *ctx, int device) {
     snprintf(ctx->name, sizeof(ctx->name), "%s (%s)", prop.name, prop.gcnArchName);
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(ctx, CSE_ERR_HIP, "device %d is %s; this engine is built for gfx950", device, prop.gcnArchName);
+    if (const char *ft = getenv("CSE_FLUSH_TERMS")) {
+        const long long v = atoll(ft);
+        if (v < 1 || v > CSE_FLUSH_TERMS_MAX)
+            return fail(ctx, CSE_ERR_ARG, "CSE_FLUSH_TERMS=%s outside [1, %lld]", ft, CSE_FLUSH_TERMS_MAX);
+        ctx->flush_terms = v;
+    }
     // this device's copy of the static scratch array, once per device (with
     // the device current); a failure only disables the no-allocation call
     if (device < kMaxScratchDevices) {

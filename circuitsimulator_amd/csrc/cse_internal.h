@@ -54,6 +54,9 @@ inline __host__ __device__ long long tri_items(long long chi, long long ch, long
 // k_sweep_packed_all: its launch bound and its persistent grid.
 #ifndef CSE_EXACT_BLOCKS_PER_CU
 #define CSE_EXACT_BLOCKS_PER_CU 7
+// largest flush_terms (and the default): 2^30 terms keep every 32-bit bin
+// below 2^31
+#define CSE_FLUSH_TERMS_MAX (1ll << 30)
 #endif
 
 // Histogram bins per instance: P in [-(64 + 64), 0], 8 phases (the public
@@ -88,6 +91,10 @@ struct SweepArgs {
     // CSE_QUEUE_STRIDE) by guided self-scheduling: a claim takes remaining /
     // waves_per_part slices, at least 1 and at most take_max
     long long take_max;
+    // mode 1: a wave flushes its partial histogram of an instance once it
+    // has folded this many terms of it (its LDS bins are 32-bit and a term
+    // adds at most 2 to one), so no bin can wrap however long the instance
+    long long flush_terms;
     long long first_take;
     long long static_end;
     long long part;

@@ -1206,6 +1206,7 @@ DEV void sweep_body(const SweepArgs &A) {
     const u64 lowx = lowmask(x);
     // PERSIST: the instance whose partial sums the wave holds, and those sums
     int cur = -1;
+    long long pend = 0;  // terms folded into the partial sums since the last flush
     LaneAcc pacc;
     pacc.clear();
     auto flush_instance = [&]() {
@@ -1226,6 +1227,7 @@ DEV void sweep_body(const SweepArgs &A) {
             }
         }
         pacc.clear();
+        pend = 0;
     };
 
     // mode 1: the wave's claimed slice range [s_cur, s_end).  Its first claim
@@ -1457,7 +1459,11 @@ DEV void sweep_body(const SweepArgs &A) {
         }
         if (ballot(bad) && x == 0) atomicOr(A.err, 2);
         if (x == 0) atomicAdd((unsigned long long *)&A.ips[inst], (unsigned long long)(j1 - j0));
-        if (PERSIST) continue;
+        if (PERSIST) {
+            pend += j1 - j0;
+            if (pend >= A.flush_terms) flush_instance();
+            continue;
+        }
         // flush the item (the LDS histogram atomics of this wave come first)
         __threadfence_block();
         const double kr = wave_sumf64((double)acc.sr - (double)acc.cr);
@@ -1490,7 +1496,47 @@ DEV void sweep_body(const SweepArgs &A) {
             }
         }
     }
-    if (PERSIST) flush_instance();
+    if constexpr (PERSIST) {
+        // The last flush is merged per block: the waves put their Kahan
+        // partials and instance in LDS, and the lowest wave holding each
+        // instance adds the block's histograms and partials for it to the
+        // global sums.  On a launch of one instance every wave's last flush
+        // lands on the same addresses, the two Kahan sums above all; a
+        // quarter of the atomics there shortens the end of the launch
+        // (C2: 0.67 -> 0.54 ms).
+        __shared__ int bcur[4];
+        __shared__ double bk[4][2];
+        const double kr = wave_sumf64((double)pacc.sr - (double)pacc.cr);
+        const double ki = wave_sumf64((double)pacc.si - (double)pacc.ci);
+        if (x == 0) {
+            bcur[wv] = cur;
+            bk[wv][0] = kr;
+            bk[wv][1] = ki;
+        }
+        __syncthreads();
+        bool lead = cur >= 0;
+        for (int w = 0; w < wv; w++)
+            if (bcur[w] == cur) lead = false;
+        if (lead) {
+            double sr = 0.0, si = 0.0;
+            for (int w = wv; w < 4; w++)
+                if (bcur[w] == cur) {
+                    sr += bk[w][0];
+                    si += bk[w][1];
+                }
+            if (x == 0) {
+                double *kz = A.kahan + 2 * (size_t)cur;
+                atomicAdd(&kz[0], sr);
+                atomicAdd(&kz[1], si);
+            }
+            for (int b = x; b < HB; b += 64) {
+                unsigned long long v = 0;
+                for (int w = wv; w < 4; w++)
+                    if (bcur[w] == cur) v += lhist[w][b];
+                if (v) atomicAdd((unsigned long long *)&A.hist[(size_t)cur * A.hist_stride + b], v);
+            }
+        }
+    }
 }
 
 extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_body<false, false, -1>(A); }

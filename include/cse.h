@@ -52,6 +52,11 @@ typedef struct {
     uint64_t zs[CSE_NMAX];
 } cse_projector;
 
+/* A context on a gfx950 device.  It reads CSE_FLUSH_TERMS from the
+ * environment when set: the number of terms of one instance after which a
+ * wave of the exact-mode sweep flushes its partial histogram (1 to 2^30;
+ * default 2^30, which keeps the sweep's 32-bit partial bins from wrapping).
+ * Results do not depend on it; a value outside that range fails the call. */
 int cse_create(int device, cse_ctx **ctx);
 void cse_destroy(cse_ctx *ctx);
 const char *cse_error(const cse_ctx *ctx);
