@@ -506,10 +506,10 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     const int grid = sweep_grid(ctx, pl.nitems, all8 ? 8 : all ? CSE_EXACT_BLOCKS_PER_CU : ctx->blocks_per_cu);
     A.take_max = pl.take_max;
     A.flush_terms = ctx->flush_terms;
-    // the static first claims cover about half of the slices (at most
-    // take_max per wave); the rest is cut into CSE_NQUEUES parts
+    // the static first claims cover about 1 / CSE_STATIC_DIV of the slices
+    // (at most take_max per wave); the rest is cut into CSE_NQUEUES parts
     const long long waves = 4ll * grid;
-    A.first_take = pl.nitems / (2 * waves);
+    A.first_take = pl.nitems / (CSE_STATIC_DIV * waves);
     if (A.first_take > A.take_max) A.first_take = A.take_max;
     if (A.first_take < 1) A.first_take = 1;
     A.static_end = A.first_take * waves < pl.nitems ? A.first_take * waves : pl.nitems;

@@ -54,6 +54,14 @@ inline __host__ __device__ long long tri_items(long long chi, long long ch, long
 // k_sweep_packed_all: its launch bound and its persistent grid.
 #ifndef CSE_EXACT_BLOCKS_PER_CU
 #define CSE_EXACT_BLOCKS_PER_CU 7
+// A dynamic claim takes remaining / (waves per part x CSE_GSS_DIV) slices;
+// the static first claims cover 1 / CSE_STATIC_DIV of all slices
+#ifndef CSE_GSS_DIV
+#define CSE_GSS_DIV 2
+#endif
+#ifndef CSE_STATIC_DIV
+#define CSE_STATIC_DIV 2
+#endif
 // largest flush_terms (and the default): 2^30 terms keep every 32-bit bin
 // below 2^31
 #define CSE_FLUSH_TERMS_MAX (1ll << 30)
@@ -88,8 +96,9 @@ struct SweepArgs {
     // g first takes slices [g first_take, g first_take + first_take) of
     // [0, static_end); the rest are cut into CSE_NQUEUES parts of `part`
     // slices, each handed out from its own counter (queue + q
-    // CSE_QUEUE_STRIDE) by guided self-scheduling: a claim takes remaining /
-    // waves_per_part slices, at least 1 and at most take_max
+    // CSE_QUEUE_STRIDE) by guided self-scheduling: a claim takes the
+    // part's remaining slices (read from its counter just before) /
+    // (waves_per_part x CSE_GSS_DIV), at least 1 and at most take_max
     long long take_max;
     // mode 1: a wave flushes its partial histogram of an instance once it
     // has folded this many terms of it (its LDS bins are 32-bit and a term

@@ -1238,7 +1238,7 @@ DEV void sweep_body(const SweepArgs &A) {
     // block's XCD first (blocks go to the XCDs round robin) and moves on to
     // the next part when that one is drained, so the claims of the whole
     // grid spread over the counters.  Claims are sized by guided
-    // self-scheduling from the position the wave last saw in the part, so
+    // self-scheduling from the part's counter read just before the claim, so
     // they shrink as the part drains.  A wave exits after finding every part
     // drained.
     const long long gw = (long long)blockIdx.x * 4 + wv;
@@ -1247,7 +1247,6 @@ DEV void sweep_body(const SweepArgs &A) {
     int qx = (int)(blockIdx.x % CSE_NQUEUES), tried = 0;
     long long q0 = A.static_end + (long long)qx * A.part, q1 = q0 + A.part;
     if (q1 > A.nitems) q1 = A.nitems;
-    long long seen = q0;
     while (true) {
         long long item = 0;
         if (mode == 0) {
@@ -1258,26 +1257,34 @@ DEV void sweep_body(const SweepArgs &A) {
             while (s_cur >= s_end && tried < CSE_NQUEUES) {
                 long long base = 0, take = 0;
                 if (x == 0) {
-                    take = (q1 - seen) / A.waves_per_part;
-                    if (take > A.take_max) take = A.take_max;
-                    if (take < 1) take = 1;
-                    base = q0 + (long long)atomicAdd(A.queue + (size_t)qx * CSE_QUEUE_STRIDE,
-                                                     (unsigned long long)take);
+                    // the claim is sized from the part's counter as it is
+                    // now (a plain load), so claims shrink as the part
+                    // drains; a part seen drained takes no claim
+                    unsigned long long *ctr = A.queue + (size_t)qx * CSE_QUEUE_STRIDE;
+                    const long long pos =
+                        q0 + (long long)__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (pos >= q1) {
+                        base = q1;
+                    } else {
+                        take = (q1 - pos) / (A.waves_per_part * CSE_GSS_DIV);
+                        if (take > A.take_max) take = A.take_max;
+                        if (take < 1) take = 1;
+                        base = q0 + (long long)atomicAdd(ctr, (unsigned long long)take);
+                    }
                 }
                 base = (long long)rl64((u64)base, 0);
                 take = (long long)rl64((u64)take, 0);
                 if (base < q1) {
                     s_cur = base;
                     s_end = base + take < q1 ? base + take : q1;
-                    seen = s_end;
                 } else {
-                    // this part is drained: the next one (every claim moves a
-                    // counter on, so each part drains and every wave leaves)
+                    // this part is drained: the next one (a counter only
+                    // grows and every claim moves one on, so each part
+                    // drains and every wave leaves)
                     tried++;
                     qx = qx + 1 < CSE_NQUEUES ? qx + 1 : 0;
                     q0 = A.static_end + (long long)qx * A.part;
                     q1 = q0 + A.part < A.nitems ? q0 + A.part : A.nitems;
-                    seen = q0;
                 }
             }
             if (s_cur >= s_end) break;

@@ -65,6 +65,8 @@ def test_triangle_enumeration_is_a_bijection(chi, width):
 
 
 NQ = 8  # counters of the dynamic range (CSE_NQUEUES)
+GSS_DIV = 2  # CSE_GSS_DIV
+STATIC_DIV = 2  # CSE_STATIC_DIV
 
 
 def run_queue(chis, waves, take_max, seed, width=SLICE):
@@ -72,16 +74,17 @@ def run_queue(chis, waves, take_max, seed, width=SLICE):
     random interleaving: a static first claim per wave (slices [g f0,
     g f0 + f0) of [0, static_end)), then the dynamic range cut into NQ parts
     with a counter each; a wave starts at the part of its block (block % NQ)
-    and moves to the next part when its claim lands past the part's end,
-    claims sized from the position it last saw in the part.  Returns every
-    (instance, i, j) term its segments cover."""
+    and moves to the next part when its claim lands past the part's end or
+    it reads the counter past it.  A claim is sized from the counter as read
+    just before it; another wave may claim between that read and the add.
+    Returns every (instance, i, j) term its segments cover and the claims."""
     rng = random.Random(seed)
     lay = [slice_layout(c, width) for c in chis]
     off = [0]
     for c, (ch, nch) in zip(chis, lay):
         off.append(off[-1] + tri_items(c, ch, nch))
     nitems = off[-1]
-    f0 = max(1, min(take_max, nitems // (2 * waves)))
+    f0 = max(1, min(take_max, nitems // (STATIC_DIV * waves)))
     static_end = min(f0 * waves, nitems)
     part = -(-(nitems - static_end) // NQ)
     wpp = -(-waves // NQ)
@@ -96,25 +99,32 @@ def run_queue(chis, waves, take_max, seed, width=SLICE):
         qx = (g // 4) % NQ
         q0, q1 = bounds(qx)
         state.append({"cur": g * f0, "end": min(g * f0 + f0, static_end), "qx": qx, "tried": 0,
-                      "q0": q0, "q1": q1, "seen": q0, "run": True})
-    terms = []
+                      "q0": q0, "q1": q1, "run": True, "read": None})
+    terms, claims = [], []
     while any(w["run"] for w in state):
         w = rng.choice([w for w in state if w["run"]])
         if w["cur"] >= w["end"]:
             if w["tried"] == NQ:
                 w["run"] = False
                 continue
-            take = max(1, min(take_max, (w["q1"] - w["seen"]) // wpp))
-            base = w["q0"] + counters[w["qx"]]
-            counters[w["qx"]] += take
+            if w["read"] is None:
+                # first step of a claim: read the counter (other waves may
+                # claim before this wave adds)
+                w["read"] = w["q0"] + counters[w["qx"]]
+                continue
+            pos, w["read"] = w["read"], None
+            base = w["q1"]
+            if pos < w["q1"]:
+                take = max(1, min(take_max, (w["q1"] - pos) // (wpp * GSS_DIV)))
+                base = w["q0"] + counters[w["qx"]]
+                counters[w["qx"]] += take
             if base < w["q1"]:
                 w["cur"], w["end"] = base, min(base + take, w["q1"])
-                w["seen"] = w["end"]
+                claims.append(w["end"] - w["cur"])
             else:
                 w["tried"] += 1
                 w["qx"] = (w["qx"] + 1) % NQ
                 w["q0"], w["q1"] = bounds(w["qx"])
-                w["seen"] = w["q0"]
             continue
         item = w["cur"]
         inst = max(i for i in range(len(chis)) if off[i] <= item)
@@ -124,7 +134,7 @@ def run_queue(chis, waves, take_max, seed, width=SLICE):
         w["cur"] += nsl
         j0, j1 = max(c * ch, r), min((c + nsl) * ch, chis[inst])
         terms += [(inst, r, j) for j in range(j0, j1)]
-    return terms
+    return terms, claims
 
 
 @pytest.mark.parametrize("width", [16, 64])
@@ -133,6 +143,15 @@ def run_queue(chis, waves, take_max, seed, width=SLICE):
                                                   ([8], 64, 4), ([2048], 100, 8)])
 def test_guided_claims_cover_every_term_once(chis, waves, take_max, width):
     for seed in range(2):
-        terms = run_queue(chis, waves, take_max, seed, width)
+        terms, _ = run_queue(chis, waves, take_max, seed, width)
         want = [(q, i, j) for q, c in enumerate(chis) for i in range(c) for j in range(i, c)]
         assert sorted(terms) == want
+
+
+def test_claims_shrink_to_one_slice_as_the_parts_drain():
+    # one chi = 2048 instance (C2's shape) on 64 waves: the dynamic claims
+    # run from about remaining / (waves per part x 2) down to single slices
+    _, claims = run_queue([2048], 64, 64, seed=3)
+    n = len(claims) // 10
+    assert sum(claims[:n]) > 8 * sum(claims[-n:])
+    assert claims.count(1) >= 8
