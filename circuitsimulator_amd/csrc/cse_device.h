@@ -486,8 +486,10 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
 // With nA2, nA6 the {2,6} steps with D_a = 2 and 6, nC the steps without a
 // constraint and n0 variables, the result is 2^{(n0 + nC)/2} w^m with
 // m = Q + nA2 - nA6 + sum_B c D_i over the constrained steps.  W (s60) is
-// the packed term being built: bits 29..31 are m, bits 0..27 are
-// 16 (base - n0 - nC) + 1.  Register use: s56 pivot index, then D_i << 29,
+// the term being built: bits 29..31 hold Q + nA2 + nA6 + sum_B c D_i (each
+// {2,6} step adds 1 there and its D_a top bit as the carry into bit 0, so
+// m is bits 29..31 - 2 nA6), bits 7..27 hold base - n0 - nC in units of
+// 128, and bits 0..6 hold 1 + nA6 (nA6 <= 60).  Register use: s56 pivot index, then D_i << 29,
 // s[58:59] act, s[62:63] J_a, s[64:65] eta, s[66:67] Jc, v40 = -2 << 29.  A
 // VALU instruction reads an SGPR that v_readlane wrote only two or more
 // instructions later.  Every step clears a bit of act below 60.
@@ -504,18 +506,14 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
     "s_and_b64 s[64:65], s[62:63], s[58:59]\n\t"                            \
     "s_bitcmp1_b32 s63, 30\n\t"                                             \
     "s_cbranch_scc0 L_nonA" S "%=\n\t"                                      \
-    /* D_a in {2,6}: rows in eta take J ^= eta, D -= D_a; m += 1            \
-       (D_a = 2) or -1 (D_a = 6) */                                         \
+    /* D_a in {2,6}: rows in eta take J ^= eta, D -= D_a; m += 1, and       \
+       the carry in (SCC = D_a's top bit) counts D_a = 6 in W's low bits */ \
     "v_xor_b32 %[vt0], s64, %[jlo]\n\t"                                     \
     "v_xad_u32 %[vt1], %[jhi], s65, v40\n\t"                                \
     "v_cndmask_b32_e64 %[jlo], %[jlo], %[vt0], s[64:65]\n\t"                \
     "v_cndmask_b32_e64 %[jhi], %[jhi], %[vt1], s[64:65]\n\t"                \
     "s_bitcmp1_b32 s63, 31\n\t"                                             \
-    "s_cbranch_scc1 L_a6" S "%=\n\t"                                        \
-    "s_add_u32 s60, s60, 0x20000000\n\t"                                    \
-    "s_branch L_top" S "%=\n"                                               \
-    "L_a6" S "%=:\n\t"                                                      \
-    "s_add_u32 s60, s60, 0xe0000000\n\t"                                    \
+    "s_addc_u32 s60, s60, 0x20000000\n\t"                                   \
     "s_branch L_top" S "%=\n"                                               \
     "L_nonA" S "%=:\n\t"                                                    \
     /* eta without bit 63: a constraint? */                                 \
@@ -572,13 +570,13 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
     "L_C" S "%=:\n\t"                                                       \
     "s_bitcmp1_b32 s63, 31\n\t"                                             \
     "s_cbranch_scc1 L_end%=\n\t"                                            \
-    "s_add_u32 s60, s60, 0xfffffff0\n\t"                                    \
+    "s_add_u32 s60, s60, 0xffffff80\n\t"                                    \
     CNEXT
 #define CSE_ELIM_LOOP CSE_ELIM_LOOP_BODY("", "s_branch L_top%=\n")
 
 // The loop on an AState (records of n <= 60, row 60 the end row); p and m
-// as as_expsum.  W's low field counts down from the base 128 (p = n0 + nC
-// <= 2 n0 <= 120), so p = 128 - low / 16.
+// as as_expsum.  W's p field counts down from the base 128 (p = n0 + nC
+// <= 2 n0 <= 120), so p = 128 - (bits 7..27 of W).
 DEV int as_expsum_packed(AState &s, int &p, int &m) {
     const u64 act0 = uniform64(s.act);
     u64 act = act0 | bit(CSE_END_ROW) | bit(63);
@@ -586,7 +584,7 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
     const int n0 = __popcll(act0);
     unsigned jlo = (unsigned)s.J, jhi = (unsigned)(s.J >> 32);
     unsigned W = (unsigned)__builtin_amdgcn_readfirstlane(
-        (int)(((unsigned)s.Q << 29) + 16u * (unsigned)(128 - n0) + 1u));
+        (int)(((unsigned)s.Q << 29) + 128u * (unsigned)(128 - n0) + 1u));
     unsigned vt0, vt1, vt2, vt3;
     int endv;
     asm volatile(
@@ -603,8 +601,8 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
         m = 0;
         return 0;
     }
-    p = 128 - (int)((W & 0x0fffffffu) >> 4);
-    m = (int)(W >> 29);
+    p = 128 - (int)((W & 0x0fffffffu) >> 7);
+    m = (int)((W >> 29) - 2u * ((W & 0x7fu) - 1u)) & 7;
     return 1;
 }
 

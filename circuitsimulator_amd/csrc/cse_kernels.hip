@@ -504,15 +504,15 @@ DEV u64 lrow_combo(const uint64_t *L, int k, u64 j) {
 // part fixed to 1 when outside supp(x_j) (act & h), the D planes, the
 // non-pivot columns (at most two), their column ballots over the pivot rows
 // (col0 for the lowest, col1 for the other) and h bits (hq bit 0, bit 1),
-// Q, and 16 k + 1 with k the record's dimension.
+// Q, and 128 k + 1 with k the record's dimension (W's p field and marker).
 struct PhiLConst {
     u64 actR, actH, D1, D2, nonact, col0, col1, h;
-    unsigned hq, Q, K16;
+    unsigned hq, Q, Kw;
 };
 
 // Per-lane tables of a batch: lane c holds x_j of j = jb + c, given as xv
 // (with bits 60 and 63 set), and the term-independent part of W:
-// m = Q + 2 |D1 & B| in bits 29..31 and 16 (k + |x_j| - n0) + 1 below,
+// m = Q + 2 |D1 & B| in bits 29..31 and 128 (k + |x_j| - n0) + 1 below,
 // B = act & h & ~x_j, n0 = |act & x_j| (both extra bits count in n0 and in
 // |x_j|).
 DEV void phiL_tables(const PhiLConst &C, u64 xv, unsigned &xtlo, unsigned &xthi, unsigned &ttab) {
@@ -522,13 +522,16 @@ DEV void phiL_tables(const PhiLConst &C, u64 xv, unsigned &xtlo, unsigned &xthi,
     const unsigned c1 = (unsigned)__popcll(C.D1 & C.actH & ~xv);
     xtlo = (unsigned)xv;
     xthi = (unsigned)(xv >> 32);
-    ttab = ((C.Q + 2u * c1) << 29) + 16u * (px - n0) + C.K16;
+    ttab = ((C.Q + 2u * c1) << 29) + 128u * (px - n0) + C.Kw;
 }
 
-// The packed term of a parked W word (0 stays 0).
+// The packed term 16 X + 1 + 2 m of a parked W word (0 stays 0): X from
+// bits 7..27, m from bits 29..31 less twice the count of D_a = 6 steps
+// (bits 0..6 hold 1 + that count).
 DEV int phiL_unpack(int w) {
     const unsigned u = (unsigned)w;
-    return u ? (int)((u & 0x0fffffffu) + ((u >> 29) << 1)) : 0;
+    const unsigned m = ((u >> 29) - 2u * ((u & 0x7fu) - 1u)) & 7u;
+    return u ? (int)((((u & 0x0fffffffu) >> 7) << 4) + 1u + (m << 1)) : 0;
 }
 
 // nb terms <phi_j | theta'>, j = jb .. jb + nb - 1, as raw W words (see
@@ -543,9 +546,9 @@ DEV int phiL_unpack(int w) {
 //    S = col_q & act the constraint S . z = h_q + |col_q & B| mod 2 is an
 //    empty sum (S = 0, right side 1), nothing (S = 0, right side 0), or an
 //    elimination of min(S) as a constrained step whose variable leaves the
-//    exponent (W += 16) and adds no factor; later columns then see
+//    exponent (W += 128) and adds no factor; later columns then see
 //    G_b ^= G_i on the rows b in S and h ^= G_i when c = 1;
-//  * the elimination loop from W = table + 4 |(E ^ D2) & B| in m, whose low
+//  * the elimination loop from W = table + 4 |(E ^ D2) & B| in m, whose p
 //    field ends as the packed exponent; 0 for a vanishing sum.
 // ALL = false (records with at most two non-pivot columns): the columns are
 // col0 (the lowest) and col1 with their h bits in hq, the second one updated
@@ -566,7 +569,7 @@ DEV int phiL_unpack(int w) {
 // below 61 of J_x & act, its own bit included) and whose D is in {0,4}
 // (bit 62 clear; the stored diagonal bit says the same, but the batch does
 // not rely on it) is uncoupled for the rest of the loop: each one is a C
-// step (a factor 2, W -= 16), or with D = 4 the sum vanishes.  They leave
+// step (a factor 2, W -= 128), or with D = 4 the sum vanishes.  They leave
 // act together and a second copy of the loop goes on without the check.
 // The end row and bit 63 are kept out of the batch.
 #define CSE_ELIM_LOOP_ISO \
@@ -585,7 +588,7 @@ DEV int phiL_unpack(int w) {
     "s_cbranch_scc1 L_isov%=\n\t" \
     "s_andn2_b64 s[58:59], s[58:59], s[64:65]\n\t" \
     "s_bcnt1_i32_b64 s66, s[64:65]\n\t" \
-    "s_lshl_b32 s66, s66, 4\n\t" \
+    "s_lshl_b32 s66, s66, 7\n\t" \
     "s_sub_u32 s60, s60, s66\n\t" \
     "s_branch L_top2%=\n" \
     "L_isov%=:\n\t" \
@@ -687,7 +690,7 @@ DEV int phiL_unpack(int w) {
     "s_cbranch_scc0 L_rnojii%=\n\t" \
     "v_xor_b32 %[vt3], %[vt3], %[vt0]\n" \
     "L_rnojii%=:\n\t" \
-    "s_add_u32 s60, s60, 0x10\n\t" \
+    "s_add_u32 s60, s60, 0x80\n\t" \
     "s_bitcmp1_b32 s51, 0\n\t" \
     "s_cbranch_scc0 L_rnoc%=\n\t" \
     "s_or_b32 s65, s65, 0x80000000\n\t" \
@@ -765,7 +768,7 @@ DEV int phiL_unpack(int w) {
     "s_cbranch_scc0 L_rnojii%=\n\t" \
     "v_xor_b32 %[vt3], %[vt3], %[vt0]\n" \
     "L_rnojii%=:\n\t" \
-    "s_add_u32 s60, s60, 0x10\n\t" \
+    "s_add_u32 s60, s60, 0x80\n\t" \
     "s_bitcmp1_b32 s51, 0\n\t" \
     "s_cbranch_scc0 L_rnoc%=\n\t" \
     "s_or_b32 s65, s65, 0x80000000\n\t" \
@@ -853,8 +856,8 @@ DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, uns
 // (t-1, t) with no qubit t: its constraint is x_t-1 = 0 (column t and h_t are
 // zero), and it has no coupling (pmask leaves its bit out).  Per term:
 //  * working copies of the J rows (D in bits 61..63, the end row), the G
-//    rows, h and act; W = wbase - 16 #constraints, wbase = Q in m and
-//    16 (k + t - n0) + 1 below;
+//    rows, h and act; W = wbase - 128 #constraints, wbase = Q in m and
+//    128 (k + t - n0) + 1 below;
 //  * the constraints, set bits of j from the lowest: restrictions as in
 //    PHIL_RESTRICT_ALL, with the column a ballot of the parity of the G rows
 //    on the pair and c = the parity of h on it;
@@ -878,7 +881,7 @@ DEV unsigned uni32(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane
     "s_add_u32 s46, %[jb], m0\n\t" \
     "s_and_b32 s46, s46, %[smask]\n\t" \
     "s_bcnt1_i32_b32 s47, s46\n\t" \
-    "s_lshl_b32 s47, s47, 4\n\t" \
+    "s_lshl_b32 s47, s47, 7\n\t" \
     "s_sub_u32 s60, %[wbase], s47\n\t" \
     "v_mov_b32 %[jlo], %[jlo0]\n\t" \
     "v_mov_b32 %[jhi], %[jhi0]\n\t" \
@@ -927,7 +930,7 @@ DEV unsigned uni32(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane
     "s_cbranch_scc0 L_hnojii%=\n\t" \
     "v_xor_b32 %[vt3], %[vt3], %[vt0]\n" \
     "L_hnojii%=:\n\t" \
-    "s_add_u32 s60, s60, 0x10\n\t" \
+    "s_add_u32 s60, s60, 0x80\n\t" \
     "s_bitcmp1_b32 s51, 0\n\t" \
     "s_cbranch_scc0 L_hnoc%=\n\t" \
     "s_or_b32 s65, s65, 0x80000000\n\t" \
@@ -1354,7 +1357,7 @@ DEV void sweep_body(const SweepArgs &A) {
             const unsigned size = (unsigned)((n + 1) / 2);
             const unsigned smask = (1u << size) - 1u;
             const unsigned pmask = smask & ~(unsigned)(n & 1);
-            const unsigned wbase = ((unsigned)rv.Q << 29) + 16u * (unsigned)(rv.k + n - __popcll(rv.act)) + 1u;
+            const unsigned wbase = ((unsigned)rv.Q << 29) + 128u * (unsigned)(rv.k + n - __popcll(rv.act)) + 1u;
             const u64 actR = rv.act | bit(CSE_END_ROW) | bit(63);
             const unsigned jlo0 = (unsigned)rv.J, jhi0 = (unsigned)(rv.J >> 32);
             for (long long jb = j0; jb < j1; jb += 64) {
@@ -1405,7 +1408,7 @@ DEV void sweep_body(const SweepArgs &A) {
                                       (q1m ? ((rv.h >> q1) & 1ull) << 1 : 0ull));
                 }
                 C.Q = (unsigned)rv.Q;
-                C.K16 = 16u * (unsigned)rv.k + 1u;
+                C.Kw = 128u * (unsigned)rv.k + 1u;
                 const unsigned jlo0 = (unsigned)rv.J, jhi0 = (unsigned)(rv.J >> 32);
                 // x_{jb + c} for batches at multiples of 64: the part of the
                 // low six bits of j (lane c, xlow) and that of the others

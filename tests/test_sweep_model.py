@@ -11,8 +11,8 @@ that ends at row 60 (J = 0, D = 4).
 
 This file restates that path in Python register for register: phiL_batch
 (the asm block of records with at most two non-pivot columns: per-lane
-tables of x_j and the term-independent part of the W word m * 2^29 |
-16 np + 1, fixation with one edge ballot, restrictions as constrained steps
+tables of x_j and the term-independent part of the W word m * 2^29 +
+128 np + 1, fixation with one edge ballot, restrictions as constrained steps
 with the second column updated in scalar registers, the loop, W parked and
 unpacked later), its exact-mode form (every non-pivot column outside
 supp(x_j) a restriction, columns from working G rows, the batch of empty
@@ -141,7 +141,7 @@ def iso_batch(J, act, W, count):
     """The batch of CSE_ELIM_LOOP_ISO at a term's first C step: the active
     variables whose row meets no active variable below bit 61 (its own bit
     included) and whose bit 62 is clear, the end row and bit 63 excepted,
-    leave act together with W -= 16 each; one of them with D = 4 (bit 63)
+    leave act together with W -= 128 each; one of them with D = 4 (bit 63)
     ends the loop through the vanishing exit (s56 = 0).  Returns (act, W,
     vanished)."""
     hmask = ((act >> 32) & 0x1FFFFFFF) | 0x40000000
@@ -155,7 +155,7 @@ def iso_batch(J, act, W, count):
         count["iso_vanish"] += 1
         return act, W, True
     count["iso_vars"] += popc(Z)
-    return act & ~Z, (W - 16 * popc(Z)) & U32, False
+    return act & ~Z, (W - 128 * popc(Z)) & U32, False
 
 
 def elim_loop(J, act, W, count, iso=False):
@@ -178,7 +178,8 @@ def elim_loop(J, act, W, count, iso=False):
                 lo = (J[x] & U32) ^ (eta & U32)
                 h2 = (((J[x] >> 32) ^ (eta >> 32)) + 0xC0000000) & U32
                 J[x] = (h2 << 32) | lo
-            W = (W + (0xE0000000 if (hi >> 31) & 1 else 0x20000000)) & U32
+            # s_addc: m += 1, and D_a's top bit (SCC) counted in bits 0..6
+            W = (W + 0x20000000 + ((hi >> 31) & 1)) & U32
             continue
         if not (eta << 1) & U64:                 # no constraint
             if (hi >> 31) & 1:                   # D_a = 4, or the end row
@@ -188,7 +189,7 @@ def elim_loop(J, act, W, count, iso=False):
                 return a, W
             count["C"] += 1
             count["w_C"] += 1
-            W = (W + 0xFFFFFFF0) & U32
+            W = (W + 0xFFFFFF80) & U32
             if iso:
                 iso = False
                 count["w_iso"] += 2 * popc(act & LIVE)
@@ -240,7 +241,7 @@ def restrict_step(J, S, c, W):
     """The restriction's elimination of i = min(S) over Sp = S with z_i = c
     (the asm's copy of the constrained step, S without bit 63): rows in Sp
     take J ^= Jc and D += D_i + 4 J_xi, rows in M take J ^= Sp (D += 4 when
-    c), W += 16 (the variable leaves the exponent) + c D_i."""
+    c), W += 128 (the variable leaves the exponent) + c D_i."""
     i = low(S)
     Jr = J[i]
     s66, s67 = Jr & U32, Jr >> 32
@@ -256,7 +257,7 @@ def restrict_step(J, S, c, W):
             h2 = (h2 + (s56 ^ (0x80000000 if (jc >> x) & 1 else 0))) & U32
         J[x] = (h2 << 32) | lo
     M = jc ^ (S if (s56 >> 30) & 1 else 0)
-    W = (W + 0x10) & U32
+    W = (W + 0x80) & U32
     if c:
         s65 |= 0x80000000
         W = (W + s56) & U32
@@ -266,8 +267,13 @@ def restrict_step(J, S, c, W):
 
 
 def unpack(W):
-    """phiL_unpack: the packed term of a parked W word (0 stays 0)."""
-    return ((W & 0x0FFFFFFF) + ((W >> 29) << 1)) & U32 if W else 0
+    """phiL_unpack: the packed term of a parked W word (0 stays 0): X from
+    bits 7..27, m from bits 29..31 less twice the D_a = 6 count that bits
+    0..6 hold above 1."""
+    if not W:
+        return 0
+    m = ((W >> 29) - 2 * ((W & 0x7F) - 1)) & 7
+    return ((((W & 0x0FFFFFFF) >> 7) << 4) + 1 + (m << 1)) & U32
 
 
 def packed_rows(r):
@@ -286,7 +292,7 @@ def batch_consts(r, n):
                 D1=sum(1 << x for x in bits(r.act) if r.d[x] & 2),
                 D2=sum(1 << x for x in bits(r.act) if r.d[x] & 4),
                 nonact=nonact, col0=col[0], col1=col[1],
-                hq=sum(((r.h >> q) & 1) << b for b, q in enumerate(qs)), Q=r.Q & 7, K16=16 * r.k + 1)
+                hq=sum(((r.h >> q) & 1) << b for b, q in enumerate(qs)), Q=r.Q & 7, Kw=128 * r.k + 1)
 
 
 def table_entry(C, xj):
@@ -296,7 +302,7 @@ def table_entry(C, xj):
     xv = xj | (1 << END) | (1 << TOP)
     n0 = popc(C["actR"] & xv)
     c1 = popc(C["D1"] & C["actH"] & ~xv)
-    T = (((C["Q"] + 2 * c1) << 29) + 16 * (popc(xv) - n0) + C["K16"]) & U32
+    T = (((C["Q"] + 2 * c1) << 29) + 128 * (popc(xv) - n0) + C["Kw"]) & U32
     return xv, T
 
 
@@ -447,12 +453,13 @@ def fallback_ip(r, xj, n, count):
             s.h ^= s.G[i]
         s.act &= ~(1 << i)
     n0 = popc(s.act)
-    W = (((s.Q << 29) & U32) + 16 * (128 - n0) + 1) & U32
+    W = (((s.Q << 29) & U32) + 128 * (128 - n0) + 1) & U32
     s56, W = elim_loop(packed_rows(s), s.act | (1 << END) | (1 << TOP), W, count)
     if s56 != END:
         return 0
-    p = 128 - ((W & 0x0FFFFFFF) >> 4)
-    return 1 + 2 * ((r.k + popc(xj) - p) * 8 + (W >> 29))
+    p = 128 - ((W & 0x0FFFFFFF) >> 7)
+    m = ((W >> 29) - 2 * ((W & 0x7F) - 1)) & 7
+    return 1 + 2 * ((r.k + popc(xj) - p) * 8 + m)
 
 
 def decode(v):
@@ -789,7 +796,7 @@ def batch_ip_H(r, j, t, count, iso=True):
     batch_ip_gen), then the couplings, clear pair bits from the lowest (on
     the working rows: Q += 4 h_u h_v, D += 4 on (cu & hv) ^ (cv & hu) ^
     (cu & cv), J ^= cv on cu and cu on cv, cu and cv the column ballots of u
-    and v), then the loop.  W starts at 16 (k + t - #constraints - n0) + 1
+    and v), then the loop.  W starts at 128 (k + t - #constraints - n0) + 1
     with Q in m.  Returns the parked W word."""
     size = (t + 1) // 2
     odd = t & 1
@@ -798,7 +805,7 @@ def batch_ip_H(r, j, t, count, iso=True):
     act = r.act | (1 << END) | (1 << TOP)
     n0 = popc(r.act)
     jm = j & ((1 << size) - 1)
-    W = (((r.Q & 7) << 29) + 16 * (r.k + t - popc(jm) - n0) + 1) & U32
+    W = (((r.Q & 7) << 29) + 128 * (r.k + t - popc(jm) - n0) + 1) & U32
     for b in bits(jm):
         q = size - 1 - b
         xi = (1 << (t - 1)) if (odd and q == size - 1) else (3 << (2 * q))
