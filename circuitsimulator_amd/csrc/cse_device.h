@@ -497,15 +497,27 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
 // (two copies can share an asm block) and CNEXT the instruction that
 // follows a step without a constraint (the branch back to the top).
 #define CSE_END_ROW 60
-#define CSE_ELIM_LOOP_BODY(S, CNEXT)                                        \
-    "L_top" S "%=:\n\t"                                                     \
+// The loop's head: the lowest active variable a leaves act, its row J_a
+// (D_a in the top bits) is read, eta = J_a & act, SCC = D_a in {2,6}.  Each
+// step ends with its own copy of the head, so a step goes on to the next
+// one with one branch (to L_A, or on to L_nonA) instead of a jump back to a
+// shared head and a second dispatch.
+#define CSE_ELIM_HEAD                                                       \
     "s_ff1_i32_b64 s56, s[58:59]\n\t"                                       \
     "s_bitset0_b64 s[58:59], s56\n\t"                                       \
     "v_readlane_b32 s62, %[jlo], s56\n\t"                                   \
     "v_readlane_b32 s63, %[jhi], s56\n\t"                                   \
     "s_and_b64 s[64:65], s[62:63], s[58:59]\n\t"                            \
-    "s_bitcmp1_b32 s63, 30\n\t"                                             \
-    "s_cbranch_scc0 L_nonA" S "%=\n\t"                                      \
+    "s_bitcmp1_b32 s63, 30\n\t"
+#define CSE_ELIM_NEXT(S)                                                    \
+    CSE_ELIM_HEAD                                                           \
+    "s_cbranch_scc1 L_A" S "%=\n\t"                                          \
+    "s_branch L_nonA" S "%=\n"
+#define CSE_ELIM_LOOP_BODY(S, CNEXT)                                        \
+    "L_top" S "%=:\n\t"                                                     \
+    CSE_ELIM_HEAD                                                           \
+    "s_cbranch_scc0 L_nonA" S "%=\n"                                        \
+    "L_A" S "%=:\n\t"                                                       \
     /* D_a in {2,6}: rows in eta take J ^= eta, D -= D_a; m += 1, and       \
        the carry in (SCC = D_a's top bit) counts D_a = 6 in W's low bits */ \
     "v_xor_b32 %[vt0], s64, %[jlo]\n\t"                                     \
@@ -514,7 +526,8 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
     "v_cndmask_b32_e64 %[jhi], %[jhi], %[vt1], s[64:65]\n\t"                \
     "s_bitcmp1_b32 s63, 31\n\t"                                             \
     "s_addc_u32 s60, s60, 0x20000000\n\t"                                   \
-    "s_branch L_top" S "%=\n"                                               \
+    CSE_ELIM_HEAD                                                           \
+    "s_cbranch_scc1 L_A" S "%=\n"                                           \
     "L_nonA" S "%=:\n\t"                                                    \
     /* eta without bit 63: a constraint? */                                 \
     "s_lshl_b64 s[66:67], s[64:65], 1\n\t"                                  \
@@ -546,7 +559,7 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
     "L_noc0" S "%=:\n\t"                                                    \
     "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"              \
     "v_bitop3_b32 %[jhi], %[jhi], %[vt1], %[vt3] bitop3:0x78\n\t"           \
-    "s_branch L_top" S "%=\n"                                               \
+    CSE_ELIM_NEXT(S)                                                        \
     /* D_i in {2,6}: M = Jc ^ Sp; rows in Sp take J ^= Jc and               \
        D += D_i + 4 J_xi */                                                 \
     "L_jii" S "%=:\n\t"                                                     \
@@ -564,7 +577,7 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
     "L_noc1" S "%=:\n\t"                                                    \
     "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"              \
     "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t"              \
-    "s_branch L_top" S "%=\n"                                               \
+    CSE_ELIM_NEXT(S)                                                        \
     /* D_a in {0,4} without a constraint: the sum vanishes (D_a = 4,        \
        or the end row) or nC += 1 */                                        \
     "L_C" S "%=:\n\t"                                                       \
