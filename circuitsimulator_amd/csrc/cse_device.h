@@ -490,7 +490,8 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
 // {2,6} step adds 1 there and its D_a top bit as the carry into bit 0, so
 // m is bits 29..31 - 2 nA6), bits 7..27 hold base - n0 - nC in units of
 // 128, and bits 0..6 hold 1 + nA6 (nA6 <= 60).  Register use: s56 pivot index, then D_i << 29,
-// s[58:59] act, s[62:63] J_a, s[64:65] eta, s[66:67] Jc, v40 = -2 << 29.  A
+// s[58:59] act, s[62:63] J_a, s[64:65] eta, s[66:67] Jc, v40 = -2 << 29,
+// v41 = 1 << 31.  A
 // VALU instruction reads an SGPR that v_readlane wrote only two or more
 // instructions later.  Every step clears a bit of act below 60.
 // CSE_ELIM_LOOP_BODY(S, CNEXT) is the loop with its labels suffixed by S
@@ -551,8 +552,7 @@ DEV bool pk_restrict_coord(AState &s, int q, unsigned alpha, u64 B) {
     "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"                         \
     "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"              \
     "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"              \
-    "v_and_b32 %[vt1], 0x80000000, %[vt0]\n\t"                              \
-    "v_xor_b32 %[vt1], s65, %[vt1]\n\t"                                     \
+    "v_bitop3_b32 %[vt1], s65, %[vt0], v41 bitop3:0x78\n\t"                 \
     "s_bitcmp1_b32 s63, 31\n\t"                                             \
     "s_cbranch_scc0 L_noc0" S "%=\n\t"                                      \
     "s_add_u32 s60, s60, s56\n"                                             \
@@ -601,14 +601,15 @@ DEV int as_expsum_packed(AState &s, int &p, int &m) {
     unsigned vt0, vt1, vt2, vt3;
     int endv;
     asm volatile(
-        "v_mov_b32 v40, 0xc0000000\n"
+        "v_mov_b32 v40, 0xc0000000\n\t"
+        "v_mov_b32 v41, 0x80000000\n"
         CSE_ELIM_LOOP
         "L_end%=:\n\t"
         "s_mov_b32 %[endv], s56\n\t"
         : "+{s[58:59]}"(act), "+{s60}"(W), [jlo] "+v"(jlo), [jhi] "+v"(jhi), [vt0] "=&v"(vt0),
           [vt1] "=&v"(vt1), [vt2] "=&v"(vt2), [vt3] "=&v"(vt3), [endv] "=&s"(endv)
         :
-        : "s56", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+        : "s56", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "v41", "scc");
     if (endv != CSE_END_ROW) {
         p = 0;
         m = 0;

@@ -599,6 +599,7 @@ DEV int phiL_unpack(int w) {
 #define PHIL_TERM(LOOP) \
     "v_mov_b32 %[parked], 0\n\t" \
     "v_mov_b32 v40, 0xc0000000\n\t" \
+    "v_mov_b32 v41, 0x80000000\n\t" \
     "s_mov_b32 s61, m0\n\t" \
     "s_mov_b32 m0, 0\n\t" \
     "s_nop 0\n" \
@@ -827,7 +828,7 @@ DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, uns
               [xtlo] "v"(xtlo), [xthi] "v"(xthi), [ttab] "v"(ttab), [actR] "s"(C.actR), [actH] "s"(C.actH),
               [D2] "s"(C.D2), [nonact] "s"(C.nonact), [col0] "s"(C.col0), [col1] "s"(C.col1), [hq] "s"(C.hq)
             : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
-              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "v41", "scc");
     } else {
         unsigned glo, ghi;
         asm volatile(
@@ -843,7 +844,7 @@ DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, uns
               [D2] "s"(C.D2), [nonact] "s"(C.nonact), [g0lo] "v"((unsigned)g0), [g0hi] "v"((unsigned)(g0 >> 32)),
               [h0] "s"(C.h), [ctlo] "v"((unsigned)colt), [cthi] "v"((unsigned)(colt >> 32))
             : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
-              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "v41", "scc");
     }
     return parked;
 }
@@ -875,6 +876,7 @@ DEV unsigned uni32(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane
 #define PHIH_BODY(LOOP) \
     "v_mov_b32 %[parked], 0\n\t" \
     "v_mov_b32 v40, 0xc0000000\n\t" \
+    "v_mov_b32 v41, 0x80000000\n\t" \
     "s_mov_b32 s61, m0\n\t" \
     "s_mov_b32 m0, 0\n" \
     "L_ip%=:\n\t" \
@@ -1021,7 +1023,7 @@ DEV int phiH_batch(unsigned nb, unsigned jb, unsigned jlo0, unsigned jhi0, u64 g
               [g0hi] "v"((unsigned)(g0 >> 32)), [actR] "s"(actR), [h0] "s"(h0), [wbase] "s"(wbase), [smask] "s"(smask),
               [sm1] "s"(sm1), [pmask] "s"(pmask)
             : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
-              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "v41", "scc");
     } else {
         asm volatile(
             PHIH_BODY(CSE_ELIM_LOOP)
@@ -1031,7 +1033,7 @@ DEV int phiH_batch(unsigned nb, unsigned jb, unsigned jlo0, unsigned jhi0, u64 g
               [g0hi] "v"((unsigned)(g0 >> 32)), [actR] "s"(actR), [h0] "s"(h0), [wbase] "s"(wbase), [smask] "s"(smask),
               [sm1] "s"(sm1), [pmask] "s"(pmask)
             : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
-              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "scc");
+              "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "v41", "scc");
     }
     return parked;
 }
