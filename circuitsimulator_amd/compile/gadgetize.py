@@ -123,6 +123,31 @@ def gadgetize(circuit, Mdict, y):
     qubits and y on the T ancillas, in the qubit order of the circuit
     followed by the ancillas (gadgetize.py:66-181).  Every generator is
     conjugated backwards through the circuit, all generators at once."""
+    pl, ng, size = _conjugate(circuit, Mdict, y)
+    phases = [((pl.P0 >> g) & 1) + 2 * ((pl.P1 >> g) & 1) for g in range(ng)]
+    xr, zr = _bit_rows(pl.X, ng), _bit_rows(pl.Z, ng)
+    return phases, [xr[g] for g in range(ng)], [zr[g] for g in range(ng)]
+
+
+def gadgetize_masks(circuit, Mdict, y):
+    """gadgetize as (phases, xs, zs, size) with xs[g] (zs[g]) an int whose
+    bit q is generator g's x (z) on qubit q."""
+    pl, ng, size = _conjugate(circuit, Mdict, y)
+    phases = [((pl.P0 >> g) & 1) + 2 * ((pl.P1 >> g) & 1) for g in range(ng)]
+    xs, zs = [0] * ng, [0] * ng
+    for q in range(size):
+        for planes, rows in ((pl.X, xs), (pl.Z, zs)):
+            m = planes[q]
+            while m:
+                g = (m & -m).bit_length() - 1
+                rows[g] |= 1 << q
+                m &= m - 1
+    return phases, xs, zs, size
+
+
+def _conjugate(circuit, Mdict, y):
+    """The bitplanes of gadgetize's generators after the backward pass, their
+    number and the qubit count n + len(y)."""
     n = len(standardGate(circuit.split("\n", 1)[0], lineMode=True))
     size = n + len(y)
     gens = [(q, Mdict[q]) for q in Mdict.keys()] + [(q, y[q - n]) for q in range(n, size)]
@@ -133,27 +158,54 @@ def gadgetize(circuit, Mdict, y):
         if outcome == 1:
             pl.P1 |= 1 << g
 
+    # _Planes.single and .cx inlined on local planes (this loop is most of
+    # a probability evaluation's host time)
+    X, Z = pl.X, pl.Z
+    P0, P1 = pl.P0, pl.P1
     tpos = n
     for m, gate, qubits in _steps(circuit):
         if m >= 0 and Mdict[m] == 0:
             continue  # the gate runs only on outcome 1
         if ng:
             if gate == "T":
-                pl.cx(qubits[0], tpos)
-                if y[tpos - n] == 1:
-                    pl.single("S", qubits[0])
-                for h in ("H", "S", "S", "S"):
-                    pl.single(h, tpos)
+                q, a = qubits[0], tpos
+                X[a] ^= X[q]                  # CX q -> ancilla
+                Z[q] ^= Z[a]
+                if y[a - n] == 1:             # S on q
+                    xq = X[q]
+                    P1 ^= P0 & xq
+                    P0 ^= xq
+                    Z[q] ^= xq
+                X[a], Z[a] = Z[a], X[a]       # H on the ancilla
+                P1 ^= X[a] & Z[a]
+                # S three times on the ancilla: X stays, Z and P0 flip by
+                # X once, and P1 gains X & ~P0 (P0 before the three)
+                xa = X[a]
+                P1 ^= xa & ~P0
+                P0 ^= xa
+                Z[a] ^= xa
+            elif gate == "H":
+                q = qubits[0]
+                X[q], Z[q] = Z[q], X[q]
+                P1 ^= X[q] & Z[q]
+            elif gate == "X":
+                P1 ^= Z[qubits[0]]
+            elif gate == "S":
+                q = qubits[0]
+                xq = X[q]
+                P1 ^= P0 & xq
+                P0 ^= xq
+                Z[q] ^= xq
             elif len(gate) == 1:
-                pl.single(gate, qubits[0])
+                raise KeyError(gate)
             else:
                 c, tq = qubits
-                pl.cx(c, tq)
+                X[tq] ^= X[c]
+                Z[c] ^= Z[tq]
         if gate == "T":
             tpos += 1
-    phases = [((pl.P0 >> g) & 1) + 2 * ((pl.P1 >> g) & 1) for g in range(ng)]
-    xr, zr = _bit_rows(pl.X, ng), _bit_rows(pl.Z, ng)
-    return phases, [xr[g] for g in range(ng)], [zr[g] for g in range(ng)]
+    pl.P0, pl.P1 = P0, P1
+    return pl, ng, size
 
 
 def _product_phase(x1, z1, x2, z2):

@@ -27,6 +27,24 @@ def projectors(circ, measure, verbose=False, x=None, y=None, rng=None):
     qubit indexes to 0/1 and receives the circuit's measurement outcomes;
     x (circuit measurements) and y (T ancillas) are 0/1 strings, drawn at
     random when not given (from rng, default numpy's global generator)."""
+    Mdict, y, n, t = _outcomes(circ, measure, verbose, x, y, rng)
+    G = gadgetize.gadgetize(circ, measure, y)
+    H = gadgetize.gadgetize(circ, Mdict, y)
+    return G, H, n, t
+
+
+def projectors_masks(circ, measure, verbose=False, x=None, y=None, rng=None):
+    """projectors() with G and H as (phases, xs, zs, n + t), each x and z row
+    an int (bit q for qubit q); the same draws and messages."""
+    Mdict, y, n, t = _outcomes(circ, measure, verbose, x, y, rng)
+    G = gadgetize.gadgetize_masks(circ, measure, y)
+    H = gadgetize.gadgetize_masks(circ, Mdict, y)
+    return G, H, n, t
+
+
+def _outcomes(circ, measure, verbose, x, y, rng):
+    """(Mdict, y, n, t): the circuit's measurement outcomes (filled into
+    measure as well) and the T-ancilla outcomes, drawn where not given."""
     rand = rng if rng is not None else np.random
     n = len(compilecirc.standardGate(circ.splitlines()[0], lineMode=True))
     t, Ms, MTs = gadgetize.countY(circ)
@@ -57,10 +75,7 @@ def projectors(circ, measure, verbose=False, x=None, y=None, rng=None):
         print("Measurement info:")
         print("x: " + "".join(str(measure[i]) if i in measure else "_" for i in range(n)))
         print("y: " + "".join(str(int(v)) for v in y))
-
-    G = gadgetize.gadgetize(circ, measure, y)
-    H = gadgetize.gadgetize(circ, Mdict, y)
-    return G, H, n, t
+    return Mdict, y, n, t
 
 
 def _null_space(A):
@@ -86,6 +101,36 @@ def _multiply(a, b):
     return (ph1 + ph2 + ph) % 4, (x1 + x2) % 2, (z1 + z2) % 2
 
 
+def masks(rows, width):
+    """Bit j of mask i is rows[i][j], for 0/1 rows of `width` entries."""
+    if len(rows) == 0:
+        return []
+    a = np.asarray(rows, dtype=np.uint8).reshape(len(rows), width)
+    packed = np.packbits(a, axis=1, bitorder="little")
+    return [int.from_bytes(row.tobytes(), "little") for row in packed]
+
+
+def _bit_rows(ms, width):
+    """0/1 int64 rows of `width` entries, row i the bits of ms[i]."""
+    nb = max(1, (width + 7) // 8)
+    raw = b"".join(m.to_bytes(nb, "little") for m in ms)
+    bits = np.unpackbits(np.frombuffer(raw, dtype=np.uint8).reshape(len(ms), nb), axis=1, bitorder="little")
+    return bits[:, :width].astype(np.int64)
+
+
+def _null_space_masks(cols, r):
+    """_null_space on bitmasks: rows are r-bit masks of generators, cols[c]
+    the generators with an X on circuit qubit c; the same column-by-column
+    construction and row order."""
+    X = [1 << i for i in range(r)]
+    for cm in cols:
+        good, bad = [], []
+        for v in X:
+            (bad if bin(v & cm).count("1") & 1 else good).append(v)
+        X = good + [bad[j] ^ bad[j - 1] for j in range(1, len(bad))]
+    return X
+
+
 def truncate(n, P):
     """(P', u): P projected onto |0^n> on its first n qubits, with u the
     number of generators lost (projectors.py:87-162).  Products equal to
@@ -94,19 +139,63 @@ def truncate(n, P):
     if len(phases) == 0:
         return ([], [], []), 0
     size = len(xs[0])
+    (ph, xm, zm, t), u = truncate_masks(n, (phases, masks(xs, size), masks(zs, size), size))
+    return to_arrays((ph, xm, zm, t)), u
+
+
+def truncate_masks(n, P):
+    """truncate on bitmask rows: P = (phases, xs, zs, size) with int rows;
+    returns ((phases, xs, zs, t), u) with rows over the t = size - n magic
+    qubits.  The null space and the products run as _null_space and
+    _multiply take them (the order of rows and of factors)."""
+    phases, xm, zm, size = P
+    r = len(phases)
     t = size - n
-    A = np.array(xs, dtype=int)[:, :n]
-    X = _null_space(A)
+    if r == 0:
+        return ([], [], [], t), 0
+    prods = null_products(n, phases, xm, zm)
     out_ph, out_x, out_z = [], [], []
-    for row in X:
-        gen = (0, np.zeros(size, dtype=int), np.zeros(size, dtype=int))
-        for i in range(len(row)):
-            if row[i] == 1:
-                gen = _multiply(gen, (phases[i], np.asarray(xs[i], dtype=int), np.asarray(zs[i], dtype=int)))
-        ph, x, z = gen
-        if ph == 0 and not x[n:].any() and not z[n:].any():
+    for _, ph, hx, hz in prods:
+        if ph == 0 and not hx and not hz:
             continue
         out_ph.append(ph)
-        out_x.append(x[n:])
-        out_z.append(z[n:])
-    return (out_ph, out_x, out_z), len(phases) - len(X)
+        out_x.append(hx)
+        out_z.append(hz)
+    return (out_ph, out_x, out_z, t), r - len(prods)
+
+
+def null_products(n, phases, xm, zm):
+    """For each row v of the null space of the generators' X parts on the
+    first n qubits (_null_space's rows, as r-bit masks): (v, phase, x, z) of
+    the product of the generators in v, in index order as _multiply takes
+    them, with x and z shifted down to the qubits from n on."""
+    r = len(phases)
+    cols = []
+    for c in range(n):
+        cm = 0
+        for i in range(r):
+            cm |= ((xm[i] >> c) & 1) << i
+        cols.append(cm)
+    out = []
+    for v in _null_space_masks(cols, r):
+        ph, x, z = 0, 0, 0
+        i, w = 0, v
+        while w:
+            if w & 1:
+                ph = (ph + int(phases[i]) + 2 * bin(z & xm[i]).count("1")) % 4
+                x ^= xm[i]
+                z ^= zm[i]
+            w >>= 1
+            i += 1
+        out.append((v, ph, x >> n, z >> n))
+    return out
+
+
+def to_arrays(P):
+    """(phases, xs, zs) with 0/1 int64 rows of a bitmask projector
+    (phases, xs, zs, width)."""
+    ph, xm, zm, width = P
+    if not ph:
+        return [], [], []
+    xr, zr = _bit_rows(xm, width), _bit_rows(zm, width)
+    return list(ph), list(xr), list(zr)

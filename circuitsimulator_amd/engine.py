@@ -286,6 +286,19 @@ class Engine:
         """instances: a list of Instance or an instance_array.  chunk: the
         claim cap in j (0: the engine's default; "auto": the autotuner's
         choice of kernel and cap for this call)."""
+        res, terms, ms = self._batch_exact(instances, chunk, want_terms, nterms)
+        return list(res), terms, ms
+
+    def batch_exact_values(self, instances, chunk=0):
+        """(values, kernel ms) of batch_exact: the exact FP64 value of every
+        instance as one array, read from the results in place."""
+        res, _, ms = self._batch_exact(instances, chunk, False, 0)
+        n = len(res)
+        vals = np.ndarray(shape=(n,), dtype=np.float64, buffer=res, offset=ExactResult.value.offset,
+                          strides=(ctypes.sizeof(ExactResult),)).copy() if n else np.zeros(0)
+        return vals, ms
+
+    def _batch_exact(self, instances, chunk, want_terms, nterms):
         kernel = None
         if isinstance(chunk, str):
             if chunk != "auto":
@@ -311,7 +324,7 @@ class Engine:
         finally:
             if kernel is not None and kernel != keep:
                 self.set_sweep_kernel(keep, mode=1)
-        return list(res), terms, ms.value
+        return res, terms, ms.value
 
     def batch_sampled(self, instances, chunk=0, want_samples=False):
         """instances: a contiguous sampled_instance_array -> (means, results,

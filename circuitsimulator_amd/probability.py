@@ -57,10 +57,13 @@ def printProjector(projector):
 
 
 def _engine_projector(P, t):
+    """Projector of (phases, xs, zs) with 0/1 rows, or of the bitmask form
+    (phases, xs, zs, t) with int rows"""
+    if len(P) == 4:
+        ph, xs, zs, _ = P
+        return Projector.make([int(p) for p in ph], list(xs), list(zs), t)
     ph, xs, zs = P
-    return Projector.make([int(p) for p in ph],
-                          [sum(int(b) << j for j, b in enumerate(x)) for x in xs],
-                          [sum(int(b) << j for j, b in enumerate(z)) for z in zs], t)
+    return Projector.make([int(p) for p in ph], _projectors.masks(xs, t), _projectors.masks(zs, t), t)
 
 
 def decompose(t, config, seed=0):
@@ -113,14 +116,14 @@ def decompose_many(t, config, seeds):
 
 
 def _same(Gp, Hp):
-    # probability.py:140-150
+    # probability.py:140-150, on bitmask rows: every generator of G' equals
+    # H''s at the same index (G' no longer than H')
     same = True
     for i in range(len(Gp[0])):
         if i > len(Hp[0]) - 1:
             same = False
             continue
-        same = (same and Gp[0][i] == Hp[0][i] and np.array_equal(Gp[1][i], Hp[1][i])
-                and np.array_equal(Gp[2][i], Hp[2][i]))
+        same = same and Gp[0][i] == Hp[0][i] and Gp[1][i] == Hp[1][i] and Gp[2][i] == Hp[2][i]
     return same
 
 
@@ -190,26 +193,27 @@ def probability(circ, measure, config={}):
         if config.get("quiet") and config.get("fidelity"):
             config["fidelity"] = False
 
-    G, H, n, t = _projectors.projectors(circ, measure, verbose=verbose, x=config.get("x"), y=config.get("y"),
-                                        rng=config.get("rng"))
-    Gprime, u = _projectors.truncate(n, G)
-    Hprime, v = _projectors.truncate(n, H)
+    # the projectors as bitmask rows (arrays only for printing and payloads)
+    G, H, n, t = _projectors.projectors_masks(circ, measure, verbose=verbose, x=config.get("x"),
+                                              y=config.get("y"), rng=config.get("rng"))
+    Gprime, u = _projectors.truncate_masks(n, G)
+    Hprime, v = _projectors.truncate_masks(n, H)
 
     if verbose and not config.get("silenceprojectors"):
         print("Gadgetize circuit:")
         print("G:")
-        printProjector(G)
+        printProjector(_projectors.to_arrays(G))
         print("H:")
-        printProjector(H)
+        printProjector(_projectors.to_arrays(H))
         print("Truncate projectors to magic state space:")
         print("Gprime: (truncated by %d)" % u)
-        printProjector(Gprime)
+        printProjector(_projectors.to_arrays(Gprime))
         print("Hprime: (truncated by %d)" % v)
-        printProjector(Hprime)
+        printProjector(_projectors.to_arrays(Hprime))
 
     # -I in the numerator: the outcome is impossible
     for i in range(len(Gprime[0])):
-        if Gprime[0][i] == 2 and not np.any(Gprime[1][i]) and not np.any(Gprime[2][i]):
+        if Gprime[0][i] == 2 and not Gprime[1][i] and not Gprime[2][i]:
             if verbose:
                 print("Found negative identity. Answer is 0.")
             return 0
@@ -232,7 +236,7 @@ def probability(circ, measure, config={}):
     if config.get("file"):
         from .protocol import make_payload
         with open(config["file"], "w") as f:
-            f.write(make_payload(Gprime, Hprime, config, t=t))
+            f.write(make_payload(_projectors.to_arrays(Gprime), _projectors.to_arrays(Hprime), config, t=t))
         if not quiet:
             print("Wrote instruction file to %s." % config["file"])
         return None

@@ -94,20 +94,13 @@ def _products(n, rows):
     phase plus 2 for every generator in the row whose outcome is 1."""
     phases, xs, zs = rows
     size = len(xs[0])
-    A = np.array(xs, dtype=int)[:, :n]
-    X = _projectors._null_space(A)
-    ph, xm, zm = [], [], []
-    for row in X:
-        gen = (0, np.zeros(size, dtype=int), np.zeros(size, dtype=int))
-        for i in range(len(row)):
-            if row[i] == 1:
-                gen = _projectors._multiply(gen, (phases[i], np.asarray(xs[i], dtype=int),
-                                                  np.asarray(zs[i], dtype=int)))
-        ph.append(gen[0])
-        xm.append(sum(int(b) << j for j, b in enumerate(gen[1][n:])))
-        zm.append(sum(int(b) << j for j, b in enumerate(gen[2][n:])))
-    X = np.array(X, dtype=int).reshape(len(X), len(phases))
-    return np.array(ph, dtype=int), xm, zm, X, len(phases) - len(X)
+    r = len(phases)
+    prods = _projectors.null_products(n, phases, _projectors.masks(xs, size), _projectors.masks(zs, size))
+    ph = [p for _, p, _, _ in prods]
+    xm = [x for _, _, x, _ in prods]
+    zm = [z for _, _, _, z in prods]
+    X = np.array([[(v >> i) & 1 for i in range(r)] for v, _, _, _ in prods], dtype=int).reshape(len(prods), r)
+    return np.array(ph, dtype=int), xm, zm, X, r - len(prods)
 
 
 def _shot_projector(prods, values):
@@ -335,8 +328,7 @@ def sampleShots(circ, measure, sample, shots, config={}):
                 fill(A, slice(nnum, 2 * nnum), [Hm[0]], Hm[1], Hm[2], False, k, Ls)
         if engine is None:
             engine = shared_engine(device)
-        res, _, _ = engine.batch_exact(A)
-        vals = np.array([r.value for r in res])
+        vals, _ = engine.batch_exact_values(A)
         num = vals[:nnum][inv]
         if not Hm[0]:
             den = np.full(len(todo), norm * norm)

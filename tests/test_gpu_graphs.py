@@ -111,3 +111,17 @@ def test_sampled_batches_replayed_match_eager(engines):
         ra, _ = g.sampled_projector(P, L, k, 0, 7 + rep, 0, 64)
         rb, _ = e.sampled_projector(P, L, k, 0, 7 + rep, 0, 64)
         assert ra.mean == rb.mean and ra.ips == rb.ips
+
+
+def test_batch_exact_values_read_in_place(engines):
+    # batch_exact_values reads every instance's value from the results array
+    # in place: the same values as batch_exact's result objects, replayed and
+    # eager
+    from circuitsimulator_amd.workloads import uneven_instances
+    g, e = engines
+    inst, _ = uneven_instances(40, seed=91, kmax=9)
+    for E in (g, e, g):
+        res, _, _ = E.batch_exact(inst)
+        vals, ms = E.batch_exact_values(inst)
+        assert vals.dtype == np.float64 and vals.shape == (len(inst),) and ms > 0
+        assert np.array_equal(vals, np.array([r.value for r in res]))
