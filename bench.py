@@ -554,6 +554,8 @@ def main():
                     help="item size in j (exact mode: the claim cap), 0 for the engine's rule, or auto for the "
                          "autotuner's choice (c4: kernel and item size) (default: auto for c2 and c3, 0 otherwise)")
     ap.add_argument("--plain-pairs", type=int, default=2048)
+    ap.add_argument("--general-pairs", type=int, default=65536,
+                    help="pairs of the general kernel's throughput leg (one launch)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5", "h30", "hs40"], default="c4",
                     help="c4: the headline norm estimation (default); c1, c2, c3, c5: the other configurations; "
@@ -660,6 +662,13 @@ def main():
         _, plain_ms = E.inner_products(t, A, B, one_launch_per_pair=True)
         _, general_batched_ms = E.inner_products(t, A, B, one_launch_per_pair=False)
         plain_ips = npairs / (plain_ms / 1000.0)
+        # the same general kernel on enough pairs to fill the device (the
+        # plain leg's pairs repeated): its throughput, not one wave's latency
+        gen_pairs = max(npairs, args.general_pairs)
+        reps = -(-gen_pairs // npairs)
+        Ag, Bg = np.concatenate([A] * reps)[:gen_pairs], np.concatenate([B] * reps)[:gen_pairs]
+        E.inner_products(t, Ag[:npairs], Bg[:npairs], one_launch_per_pair=False)  # warm
+        _, general_full_ms = E.inner_products(t, Ag, Bg, one_launch_per_pair=False)
 
         # untimed: every term of timed step 0's first draws, for the parity leg
         gpu_check = None
@@ -689,6 +698,8 @@ def main():
         "plain_ip_per_s": plain_ips,
         "plain_pairs": npairs,
         "general_batched_ip_per_s": npairs / (general_batched_ms / 1000.0),
+        "general_full_ip_per_s": gen_pairs / (general_full_ms / 1000.0),
+        "general_full_pairs": gen_pairs,
         "batched_vs_plain": value / plain_ips,
         "estimator": {"int64_exact": res.mean, "kahan_fp32": res.mean_kahan,
                       "exact_value_rounded_once": exact_last,
