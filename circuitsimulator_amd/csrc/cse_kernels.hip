@@ -1038,6 +1038,69 @@ DEV int phiH_batch(unsigned nb, unsigned jb, unsigned jlo0, unsigned jhi0, u64 g
     return parked;
 }
 
+// The pair operations that every term of a phiH_batch batch shares: those
+// of the bits of j outside the batch's varying low bits, applied once per
+// batch to the record's rows, as phiH_batch applies them per term (the
+// same operations in the same order for these bits: constraints for the
+// set bits of cbits, then couplings for the bits of pbits, each from the
+// lowest; the order of pair operations does not change a term's value).
+// J: packed rows (D in bits 61..63), G: rows, h, act, W as phiH_batch keeps
+// them; W comes in as the batch's wbase and leaves less 128 per shared
+// constraint plus the operations' terms.  Returns false when a shared
+// constraint is empty with right side 1: every term of the batch is zero.
+DEV bool phiH_prefix(unsigned cbits, unsigned pbits, unsigned sm1, u64 &J, u64 &G, u64 &h, u64 &act,
+                     unsigned &W) {
+    const int x = lane_id();
+    W -= 128u * (unsigned)__popc(cbits);
+    while (cbits) {
+        const int b = __builtin_ctz(cbits);
+        cbits &= cbits - 1;
+        const u64 pm = 3ull << (2 * ((int)sm1 - b));
+        const unsigned c = (unsigned)__popcll(h & pm) & 1u;
+        const u64 S = ballot(__popcll(G & pm) & 1) & act;
+        if (!S) {
+            if (c) return false;
+            continue;
+        }
+        const int i = __builtin_ctzll(S);
+        act &= ~bit(i);
+        const u64 Ji = rl64(J, i), Gi = rl64(G, i);
+        const unsigned s56 = (unsigned)(Ji >> 32) & 0xe0000000u;
+        const u64 Jc = Ji & ~(7ull << 61);
+        const bool inS = (S >> x) & 1ull, inJc = (Jc >> x) & 1ull;
+        if (inS) {
+            // J ^= Jc, D += D_i + 4 J_xi, G ^= G_i (row i retires)
+            const u64 Jn = J ^ Jc;
+            const unsigned hi = (unsigned)(Jn >> 32) + (s56 ^ (inJc ? 0x80000000u : 0u));
+            J = (Jn & 0xffffffffull) | ((u64)hi << 32);
+            G ^= Gi;
+        }
+        // M = Jc, or Jc ^ S when D_i is in {2,6}
+        const bool inM = inJc != (((s56 >> 30) & 1u) && inS);
+        W += 0x80u;
+        u64 Sx = S;
+        if (c) {
+            Sx |= bit(63);
+            W += s56;
+            h ^= Gi;
+        }
+        if (inM) J ^= Sx;
+    }
+    while (pbits) {
+        const int b = __builtin_ctz(pbits);
+        pbits &= pbits - 1;
+        const int u = 2 * ((int)sm1 - b);
+        const u64 cu = ballot((G >> u) & 1ull) & act, cv = ballot((G >> (u + 1)) & 1ull) & act;
+        const unsigned hh = (unsigned)(h >> u) & 3u;
+        if (hh == 3u) W += 0x80000000u;
+        const u64 lin = ((hh & 1u) ? cv : 0ull) ^ ((hh & 2u) ? cu : 0ull) ^ (cu & cv);
+        if ((lin >> x) & 1ull) J ^= 1ull << 63;
+        if ((cu >> x) & 1ull) J ^= cv;
+        if ((cv >> x) & 1ull) J ^= cu;
+    }
+    return true;
+}
+
 // <phi_j | theta'> for phi_j = prepH(j), packed: pair constraints
 // x_{2q} = x_{2q+1} (bit 1), couplings 4 x_{2q} x_{2q+1} (bit 0), last qubit
 // of odd t fixed to 0 on bit 1 (stateprep.c:36-81, as a coordinate-form
@@ -1355,18 +1418,27 @@ DEV void sweep_body(const SweepArgs &A) {
             bad |= fold_terms(parked, nb, mode, diag, rv.sexp, acc, hist, PERSIST ? rv.sexp * 8 : 0);
         };
         if (D.exact && PK) {
-            // the whole batch in one asm block
+            // the pair operations the batch shares once, then the whole
+            // batch in one asm block
             const unsigned size = (unsigned)((n + 1) / 2);
             const unsigned smask = (1u << size) - 1u;
             const unsigned pmask = smask & ~(unsigned)(n & 1);
             const unsigned wbase = ((unsigned)rv.Q << 29) + 128u * (unsigned)(rv.k + n - __popcll(rv.act)) + 1u;
             const u64 actR = rv.act | bit(CSE_END_ROW) | bit(63);
-            const unsigned jlo0 = (unsigned)rv.J, jhi0 = (unsigned)(rv.J >> 32);
             for (long long jb = j0; jb < j1; jb += 64) {
                 const int nb = (int)(j1 - jb < 64 ? j1 - jb : 64);
-                const int parked = phiL_unpack(phiH_batch<ALL>(uni32((unsigned)nb), uni32((unsigned)jb), jlo0, jhi0, rv.G,
-                                                          actR, rv.h, uni32(wbase), uni32(smask), uni32(size - 1u),
-                                                          uni32(pmask)));
+                // the bits of j the batch's terms agree on: their pair
+                // operations run once, in phiH_prefix
+                const unsigned d = (unsigned)(jb ^ (jb + nb - 1));
+                const unsigned vary = d ? (2u << (31 - __builtin_clz(d))) - 1u : 0u;
+                u64 J = rv.J, G = rv.G, h = rv.h, act = actR;
+                unsigned W = wbase;
+                int parked = 0;
+                if (phiH_prefix((unsigned)jb & smask & ~vary, pmask & ~(unsigned)jb & ~vary, size - 1u, J, G, h,
+                                act, W))
+                    parked = phiL_unpack(phiH_batch<ALL>(uni32((unsigned)nb), uni32((unsigned)jb), (unsigned)J,
+                                                         (unsigned)(J >> 32), G, uniform64(act), uniform64(h), uni32(W),
+                                                         uni32(smask & vary), uni32(size - 1u), uni32(pmask & vary)));
                 flush(parked, nb, jb);
             }
         } else if (D.exact) {

@@ -135,7 +135,8 @@ def _exact_mean(t, sexp, zsum):
     return float(tot / len(sexp))
 
 
-@pytest.mark.parametrize("t,r,k,exact", [(6, 3, 3, 0), (8, 4, 4, 0), (7, 2, 0, 1), (10, 5, 5, 0), (12, 6, 0, 1)])
+@pytest.mark.parametrize("t,r,k,exact", [(6, 3, 3, 0), (8, 4, 4, 0), (7, 2, 0, 1), (10, 5, 5, 0), (12, 6, 0, 1),
+                                         (14, 6, 0, 1), (15, 5, 0, 1)])
 def test_sampled_small_all_terms(E, t, r, k, exact):
     from circuitsimulator_amd.engine import Projector
     rng = np.random.default_rng(100 * t + r)
@@ -183,6 +184,48 @@ def test_exact_all_terms(E, t, r, exact, k):
     assert res[0].im == 0.0
     assert abs(res[0].value - abs(tot)) <= 1e-12 * max(1, abs(tot))
     assert abs(res[0].kahan_re - tot.real) <= 1e-5 * max(1, abs(tot))
+
+
+def _exact_instance_terms(E, t, ph, xs, zs, chunk):
+    from circuitsimulator_amd.engine import Instance, Projector
+    inst = Instance()
+    inst.P = Projector.make(ph, xs, zs, t)
+    inst.exact = 1
+    inst.k = 0
+    chi = 1 << ((t + 1) // 2)
+    nl = chi * (chi + 1) // 2
+    res, terms, _ = E.batch_exact([inst], chunk=chunk, want_terms=True, nterms=nl)
+    want = oracle_exact_terms(t, 0, [], 1, ph, xs, zs)
+    assert [w[0] for w in want] == list(terms)
+    tot = 0
+    for (pt, s, i, j) in want:
+        if pt:
+            v = term_value(pt, s)
+            tot += v if i == j else 2 * v.real
+    assert abs(res[0].re - tot.real) <= 1e-12 * max(1, abs(tot))
+    assert res[0].im == 0.0
+    return terms
+
+
+@pytest.mark.parametrize("t,r,chunk", [(13, 5, 0), (14, 6, 64), (15, 4, 0), (16, 7, 128)])
+def test_exact_H_terms_with_shared_pair_operations(E, t, r, chunk):
+    # |H^t> terms at t >= 13, where a batch's terms share the pair bits
+    # above their six low ones: those pair operations run once per batch
+    # (phiH_prefix), for aligned batches and for each record's first,
+    # unaligned one (from its diagonal); every term against the oracle
+    rng = np.random.default_rng(900 + t)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    print(f"exact |H^t> terms t={t} r={r} chunk={chunk}", flush=True)
+    _exact_instance_terms(E, t, ph, xs, zs, chunk)
+
+
+def test_exact_H_terms_when_a_shared_constraint_vanishes(E):
+    # -Z_0 Z_1 fixes x_0 + x_1 = 1: every j whose highest pair bit (pair 0)
+    # is set gives zero, and in aligned batches that bit is shared, so the
+    # batch's shared constraint vanishes for all 64 terms
+    t = 16
+    terms = _exact_instance_terms(E, t, [2, 0, 0], [0, 0, 0b110000], [0b11, 0b1100, 0], 0)
+    assert np.count_nonzero(terms) > 0 and np.count_nonzero(terms == 0) > 0
 
 
 def test_batch_uneven_instances(E):

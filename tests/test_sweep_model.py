@@ -786,27 +786,21 @@ def test_direct_rref_projection_branches():
         assert count[kind] > 0, (kind, dict(count))
 
 
-def batch_ip_H(r, j, t, count, iso=True):
-    """One |H^t> term of phiH_batch: <prepH(j) | theta'> from the record r
-    (stateprep.c:36-81: pair q = (2q, 2q+1) is controlled by bit size-1-q of
-    j; bit 1 makes it |00> + |11>, the constraint x_2q = x_2q+1, bit 0 the
-    four-term state, the coupling 4 x_2q x_2q+1; for odd t the last qubit is
-    |0> on bit 1 and |+> on bit 0).  The constraints come first, set bits of
-    j from the lowest (restrictions on working G rows and h, as in
-    batch_ip_gen), then the couplings, clear pair bits from the lowest (on
-    the working rows: Q += 4 h_u h_v, D += 4 on (cu & hv) ^ (cv & hu) ^
-    (cu & cv), J ^= cv on cu and cu on cv, cu and cv the column ballots of u
-    and v), then the loop.  W starts at 128 (k + t - #constraints - n0) + 1
-    with Q in m.  Returns the parked W word."""
+def h_ops(J, G, h, act, W, cbits, pbits, t, count):
+    """phiH_batch's pair operations on a working state, in place on the row
+    lists J and G: constraints for the set bits of cbits, then couplings
+    for the bits of pbits, each from the lowest bit (pair q = size-1-bit,
+    stateprep.c:36-81: bit 1 makes it |00> + |11>, the constraint
+    x_2q = x_2q+1, bit 0 the four-term state, the coupling 4 x_2q x_2q+1;
+    for odd t the last qubit is |0> on bit 1 and |+> on bit 0).  A
+    constraint is a restriction on the working G rows and h (as in
+    batch_ip_gen); a coupling, on the working rows, is Q += 4 h_u h_v,
+    D += 4 on (cu & hv) ^ (cv & hu) ^ (cu & cv), J ^= cv on cu and cu on cv
+    (cu, cv the column ballots of u and v).  Returns (h, act, W), or None
+    when a constraint is empty with right side 1 (the term vanishes)."""
     size = (t + 1) // 2
     odd = t & 1
-    J = packed_rows(r)
-    G, h = r.G[:], r.h
-    act = r.act | (1 << END) | (1 << TOP)
-    n0 = popc(r.act)
-    jm = j & ((1 << size) - 1)
-    W = (((r.Q & 7) << 29) + 128 * (r.k + t - popc(jm) - n0) + 1) & U32
-    for b in bits(jm):
+    for b in bits(cbits):
         q = size - 1 - b
         xi = (1 << (t - 1)) if (odd and q == size - 1) else (3 << (2 * q))
         count["h_restrict"] += 1
@@ -816,7 +810,7 @@ def batch_ip_H(r, j, t, count, iso=True):
         if not S:
             if c:
                 count["h_vanish"] += 1
-                return 0
+                return None
             continue
         i = low(S)
         act &= ~(1 << i)
@@ -826,8 +820,7 @@ def batch_ip_H(r, j, t, count, iso=True):
         if c:
             h ^= Gi
         W = restrict_step(J, S, c, W)
-    pairs = ((1 << size) - 1) & ~(1 if odd else 0)
-    for b in bits(~jm & pairs):
+    for b in bits(pbits):
         q = size - 1 - b
         u, v = 2 * q, 2 * q + 1
         count["h_couple"] += 1
@@ -843,8 +836,123 @@ def batch_ip_H(r, j, t, count, iso=True):
             J[x] ^= cv
         for x in bits(cv):
             J[x] ^= cu
-    s56, W = elim_loop(J, act, W, count, iso=iso)
+    return h, act, W
+
+
+def h_masks(t):
+    size = (t + 1) // 2
+    smask = (1 << size) - 1
+    return smask, smask & ~(1 if t & 1 else 0)
+
+
+def prefix_H(r, jb, vary, t, count):
+    """phiH_prefix: the pair operations every term of a batch shares, those
+    of the bits of jb outside vary, on the record's rows; W starts at
+    128 (k + t - n0) + 1 with Q in m, less 128 per shared constraint.
+    Returns the working state (J, G, h, act, W), or None when a shared
+    constraint vanishes (every term of the batch is zero)."""
+    smask, pairs = h_masks(t)
+    J, G, h = packed_rows(r), r.G[:], r.h
+    act = r.act | (1 << END) | (1 << TOP)
+    cbits = jb & smask & ~vary
+    W = (((r.Q & 7) << 29) + 128 * (r.k + t - popc(r.act)) + 1 - 128 * popc(cbits)) & U32
+    st = h_ops(J, G, h, act, W, cbits, pairs & ~jb & ~vary, t, count)
+    if st is None:
+        return None
+    h, act, W = st
+    return J, G, h, act, W
+
+
+def batch_ip_H_from(state, j, vary, t, count, iso=True):
+    """One term of phiH_batch from a batch's shared state: the pair
+    operations of the bits of j in vary, then the loop; returns the parked
+    W word."""
+    if state is None:
+        return 0
+    smask, pairs = h_masks(t)
+    J0, G0, h, act, W = state
+    J, G = J0[:], G0[:]
+    cbits = j & smask & vary
+    W = (W - 128 * popc(cbits)) & U32
+    st = h_ops(J, G, h, act, W, cbits, pairs & ~j & vary, t, count)
+    if st is None:
+        return 0
+    h, act, W = st
+    s56, W = elim_loop(J, act, W, count, iso=True if iso else False)
     return W if s56 == END else 0
+
+
+def batch_ip_H(r, j, t, count, iso=True):
+    """One |H^t> term of phiH_batch with every pair operation its own: W
+    starts at 128 (k + t - #constraints - n0) + 1 with Q in m.  Returns the
+    parked W word."""
+    smask, _ = h_masks(t)
+    return batch_ip_H_from(prefix_H(r, 0, smask, t, count), j, smask, t, count, iso=iso)
+
+
+def batch_vary(jb, nb):
+    """the bits of j that vary over [jb, jb + nb): everything below the
+    highest bit where jb and jb + nb - 1 differ"""
+    d = jb ^ (jb + nb - 1)
+    return (2 << (d.bit_length() - 1)) - 1 if d else 0
+
+
+@pytest.mark.parametrize("t,r,seed", [(13, 6, 81), (14, 5, 82), (16, 7, 83), (17, 4, 84), (20, 9, 85),
+                                      (21, 8, 86)])
+def test_phiH_batch_prefix_matches_oracle(t, r, seed):
+    # the shared pair operations of a batch (the bits of j its terms agree
+    # on) applied once, then each term's own: every term equals the oracle,
+    # for aligned batches of 64, an unaligned one (an exact-mode diagonal
+    # start), a short one and a single term
+    from circuitsimulator_amd.workloads import commuting_paulis
+    rng = np.random.default_rng(seed)
+    ph, xs, zs = commuting_paulis(rng, t, r)
+    Po = orc.make_projector(ph, xs, zs, t)
+    chi = 1 << ((t + 1) // 2)
+    thr = orc.thresholds(t)
+    count = collections.Counter()
+    states = [orc.prepH(int(i), t) for i in rng.integers(0, chi, size=3)]
+    states += [orc.random_state(t, seed, s, thr) for s in range(3)]
+    shared = 0
+    for th in states:
+        if orc.project(th, Po) < 0:
+            continue
+        rec = to_record(th)
+        for jb, nb in [(64 * int(rng.integers(0, chi // 64)), 64), (int(rng.integers(0, chi - 64)), 64),
+                       (64 * int(rng.integers(0, chi // 64)) + 5, 37), (int(rng.integers(0, chi)), 1)]:
+            vary = batch_vary(jb, nb)
+            shared += popc(((1 << ((t + 1) // 2)) - 1) & ~vary)
+            st = prefix_H(rec, jb, vary, t, count)
+            for j in range(jb, jb + nb):
+                e, p, m = orc.inner(th, orc.prepH(j, t))
+                want = (e, p, m % 8) if e else (0, 0, 0)
+                assert decode(unpack(batch_ip_H_from(st, j, vary, t, count))) == want, (t, jb, nb, j)
+    assert shared > 0 and count["h_restrict"] > 0 and count["h_couple"] > 0
+
+
+def test_phiH_batch_prefix_can_vanish():
+    # a projector holding -Z_0 Z_1 fixes x_0 + x_1 = 1, so every j whose
+    # highest pair bit (pair 0) is set gives zero: in an aligned batch that
+    # bit is shared, and the batch's prefix vanishes for all its terms
+    t = 16
+    chi = 1 << ((t + 1) // 2)
+    ph, xs, zs = [2, 0, 0], [0, 0, 0b110000], [0b11, 0b1100, 0]
+    Po = orc.make_projector(ph, xs, zs, t)
+    count = collections.Counter()
+    vanished = 0
+    for i in (3, 77, 200):
+        th = orc.prepH(i, t)
+        if orc.project(th, Po) < 0:
+            continue
+        rec = to_record(th)
+        for jb in range(0, chi, 64):
+            st = prefix_H(rec, jb, 63, t, count)
+            vanished += st is None
+            for j in range(jb, jb + 64, 7):
+                e, p, m = orc.inner(th, orc.prepH(j, t))
+                want = (e, p, m % 8) if e else (0, 0, 0)
+                assert decode(unpack(batch_ip_H_from(st, j, 63, t, count))) == want, (i, j)
+    assert vanished > 0
 
 
 @pytest.mark.parametrize("t,r,seed", [(6, 3, 71), (7, 2, 72), (12, 5, 73), (13, 6, 74), (20, 6, 75), (9, 4, 76)])
