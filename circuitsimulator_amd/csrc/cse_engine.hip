@@ -35,7 +35,7 @@ struct HostBuf {
 // contents: its sizes, kernels and grids (all functions of these) and every
 // buffer address its nodes hold.  A call whose key matches replays the graph.
 struct GraphKey {
-    int mode, ninst, packed, kernel;
+    int mode, ninst, kernel;
     long long nrec, nitems, take_max;
     size_t plan_bytes, out_bytes;
     const void *ptr[8];
@@ -480,6 +480,26 @@ int prep_batch(cse_ctx *ctx, const Plan &pl, int mode) {
     return zero_list(ctx, zl);
 }
 
+enum SweepKernel { SK_GENERIC, SK_PACKED, SK_PACKED_GEN, SK_ALL, SK_ALL8 };
+
+// The sweep kernel of a launch.  Exact mode takes k_sweep_packed_all (seven
+// waves per SIMD) or, when set, k_sweep_packed_all8.  Sampled launches take
+// k_sweep_packed (eight waves, the two-column asm path for |L> terms), or
+// k_sweep_packed_gen when set or when every term is an |H^t> term: that
+// kernel runs an aligned batch of |H^t> terms as eight groups of 8 that share
+// the pair operations of bits 3..5 of j.  Records of n > 60 take k_sweep.
+SweepKernel sweep_choice(const cse_ctx *ctx, const Plan &pl, int mode) {
+    bool packed = true, allH = true;
+    for (int i = 0; i < pl.ninst; i++) {
+        packed = packed && pl.inst[i].n <= CSE_END_ROW;
+        allH = allH && pl.inst[i].exact;
+    }
+    if (!packed) return SK_GENERIC;
+    if (mode == 1) return ctx->sweep_kernel[1] == 2 ? SK_ALL8 : SK_ALL;
+    const int k = ctx->sweep_kernel[0];
+    return k == 2 || (k == 0 && allH) ? SK_PACKED_GEN : SK_PACKED;
+}
+
 int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     const int ninst = pl.ninst;
     SweepArgs A;
@@ -497,13 +517,9 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     A.hist_stride = HIST_STRIDE;
     A.ips = ctx->d_ips;
     A.err = ctx->d_err;
-    bool packed = true;
-    for (int i = 0; i < ninst; i++) packed = packed && pl.inst[i].n <= CSE_END_ROW;
-    // exact mode takes k_sweep_packed_all (seven waves per SIMD); sampled
-    // launches, |L> and |H^t> terms alike, take k_sweep_packed (eight)
-    const bool all = packed && mode == 1;
-    const bool all8 = all && ctx->sweep_kernel[1] == 2;
-    const int grid = sweep_grid(ctx, pl.nitems, all8 ? 8 : all ? CSE_EXACT_BLOCKS_PER_CU : ctx->blocks_per_cu);
+    const SweepKernel sk = sweep_choice(ctx, pl, mode);
+    const int grid = sweep_grid(ctx, pl.nitems, sk == SK_ALL8 ? 8 : sk == SK_ALL ? CSE_EXACT_BLOCKS_PER_CU
+                                                                                 : ctx->blocks_per_cu);
     A.take_max = pl.take_max;
     A.flush_terms = ctx->flush_terms;
     // the static first claims cover about 1 / CSE_STATIC_DIV of the slices
@@ -515,12 +531,13 @@ int run_sweep(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms) {
     A.static_end = A.first_take * waves < pl.nitems ? A.first_take * waves : pl.nitems;
     A.part = (pl.nitems - A.static_end + CSE_NQUEUES - 1) / CSE_NQUEUES;
     A.waves_per_part = (waves + CSE_NQUEUES - 1) / CSE_NQUEUES;
-    if (all8) hipLaunchKernelGGL(k_sweep_packed_all8, dim3(grid), dim3(256), 0, ctx->stream, A);
-    else if (all) hipLaunchKernelGGL(k_sweep_packed_all, dim3(grid), dim3(256), 0, ctx->stream, A);
-    else if (packed && ctx->sweep_kernel[0] == 2)
-        hipLaunchKernelGGL(k_sweep_packed_gen, dim3(grid), dim3(256), 0, ctx->stream, A);
-    else if (packed) hipLaunchKernelGGL(k_sweep_packed, dim3(grid), dim3(256), 0, ctx->stream, A);
-    else hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(256), 0, ctx->stream, A);
+    switch (sk) {
+    case SK_ALL8: hipLaunchKernelGGL(k_sweep_packed_all8, dim3(grid), dim3(256), 0, ctx->stream, A); break;
+    case SK_ALL: hipLaunchKernelGGL(k_sweep_packed_all, dim3(grid), dim3(256), 0, ctx->stream, A); break;
+    case SK_PACKED_GEN: hipLaunchKernelGGL(k_sweep_packed_gen, dim3(grid), dim3(256), 0, ctx->stream, A); break;
+    case SK_PACKED: hipLaunchKernelGGL(k_sweep_packed, dim3(grid), dim3(256), 0, ctx->stream, A); break;
+    default: hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(256), 0, ctx->stream, A); break;
+    }
     HIPCHK(hipGetLastError());
     return CSE_OK;
 }
@@ -556,9 +573,7 @@ int run_batch(cse_ctx *ctx, const Plan &pl, int mode, bool want_terms, size_t pl
     memset(&key, 0, sizeof(key));
     key.mode = mode;
     key.ninst = pl.ninst;
-    key.packed = 1;
-    for (int i = 0; i < pl.ninst; i++) key.packed &= pl.inst[i].n <= CSE_END_ROW ? 1 : 0;
-    key.kernel = ctx->sweep_kernel[mode];
+    key.kernel = (int)sweep_choice(ctx, pl, mode);
     key.nrec = pl.nrec;
     key.nitems = pl.nitems;
     key.take_max = pl.take_max;

@@ -874,11 +874,10 @@ DEV int phiL_batch(unsigned nb, unsigned jlo0, unsigned jhi0, unsigned lxlo, uns
 DEV unsigned uni32(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
 
 #define PHIH_BODY(LOOP) \
-    "v_mov_b32 %[parked], 0\n\t" \
     "v_mov_b32 v40, 0xc0000000\n\t" \
     "v_mov_b32 v41, 0x80000000\n\t" \
     "s_mov_b32 s61, m0\n\t" \
-    "s_mov_b32 m0, 0\n" \
+    "s_mov_b32 m0, %[m0s]\n" \
     "L_ip%=:\n\t" \
     "s_add_u32 s46, %[jb], m0\n\t" \
     "s_and_b32 s46, s46, %[smask]\n\t" \
@@ -1009,29 +1008,31 @@ DEV unsigned uni32(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane
     "s_mov_b32 m0, s61\n\t" \
     "s_nop 1\n\t"
 
+// Terms jb + m0s .. jb + nb - 1 into lanes m0s .. nb - 1 of parked, which
+// keeps its other lanes (a batch can run as several calls, each from its own
+// shared state).
 template <bool ALL>
 DEV int phiH_batch(unsigned nb, unsigned jb, unsigned jlo0, unsigned jhi0, u64 g0, u64 actR, u64 h0,
-                   unsigned wbase, unsigned smask, unsigned sm1, unsigned pmask) {
-    int parked;
+                   unsigned wbase, unsigned smask, unsigned sm1, unsigned pmask, unsigned m0s, int parked) {
     unsigned jlo, jhi, vt0, vt1, vt2, vt3, glo, ghi;
     if constexpr (ALL) {
         asm volatile(
             PHIH_BODY(CSE_ELIM_LOOP_ISO)
-            : [parked] "=&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
+            : [parked] "+&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
               [vt2] "=&v"(vt2), [vt3] "=&v"(vt3), [glo] "=&v"(glo), [ghi] "=&v"(ghi)
             : [nb] "s"(nb), [jb] "s"(jb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [g0lo] "v"((unsigned)g0),
               [g0hi] "v"((unsigned)(g0 >> 32)), [actR] "s"(actR), [h0] "s"(h0), [wbase] "s"(wbase), [smask] "s"(smask),
-              [sm1] "s"(sm1), [pmask] "s"(pmask)
+              [sm1] "s"(sm1), [pmask] "s"(pmask), [m0s] "s"(m0s)
             : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
               "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "v41", "scc");
     } else {
         asm volatile(
             PHIH_BODY(CSE_ELIM_LOOP)
-            : [parked] "=&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
+            : [parked] "+&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
               [vt2] "=&v"(vt2), [vt3] "=&v"(vt3), [glo] "=&v"(glo), [ghi] "=&v"(ghi)
             : [nb] "s"(nb), [jb] "s"(jb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [g0lo] "v"((unsigned)g0),
               [g0hi] "v"((unsigned)(g0 >> 32)), [actR] "s"(actR), [h0] "s"(h0), [wbase] "s"(wbase), [smask] "s"(smask),
-              [sm1] "s"(sm1), [pmask] "s"(pmask)
+              [sm1] "s"(sm1), [pmask] "s"(pmask), [m0s] "s"(m0s)
             : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
               "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "v41", "scc");
     }
@@ -1435,10 +1436,30 @@ DEV void sweep_body(const SweepArgs &A) {
                 unsigned W = wbase;
                 int parked = 0;
                 if (phiH_prefix((unsigned)jb & smask & ~vary, pmask & ~(unsigned)jb & ~vary, size - 1u, J, G, h,
-                                act, W))
-                    parked = phiL_unpack(phiH_batch<ALL>(uni32((unsigned)nb), uni32((unsigned)jb), (unsigned)J,
-                                                         (unsigned)(J >> 32), G, uniform64(act), uniform64(h), uni32(W),
-                                                         uni32(smask & vary), uni32(size - 1u), uni32(pmask & vary)));
+                                act, W)) {
+                    if (ALL && vary == 63u && nb == 64) {
+                        // an aligned batch: bits 3..5 are shared by each
+                        // group of 8 terms, once per group (not in the
+                        // sampled |L> kernel: the group's copy of the
+                        // state would cost it VGPRs)
+                        for (unsigned g = 0; g < 64u; g += 8u) {
+                            u64 J8 = J, G8 = G, h8 = h, act8 = act;
+                            unsigned W8 = W;
+                            const unsigned jg = (unsigned)jb + g;
+                            if (phiH_prefix(jg & smask & 0x38u, pmask & ~jg & 0x38u, size - 1u, J8, G8, h8, act8, W8))
+                                parked = phiH_batch<ALL>(uni32(g + 8u), uni32((unsigned)jb), (unsigned)J8,
+                                                         (unsigned)(J8 >> 32), G8, uniform64(act8), uniform64(h8),
+                                                         uni32(W8), uni32(smask & 7u), uni32(size - 1u),
+                                                         uni32(pmask & 7u), uni32(g), parked);
+                        }
+                    } else {
+                        parked = phiH_batch<ALL>(uni32((unsigned)nb), uni32((unsigned)jb), (unsigned)J,
+                                                 (unsigned)(J >> 32), G, uniform64(act), uniform64(h), uni32(W),
+                                                 uni32(smask & vary), uni32(size - 1u), uni32(pmask & vary), 0u,
+                                                 parked);
+                    }
+                }
+                parked = phiL_unpack(parked);
                 flush(parked, nb, jb);
             }
         } else if (D.exact) {
@@ -1637,8 +1658,10 @@ extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed(SweepArgs A)
 // The sampled sweep with the general restriction loop of exact mode: every
 // |L> term runs in the asm block whatever the record's number of non-pivot
 // columns, every |H^t> term in phiH_batch with the batch of uncoupled
-// variables.  Eight waves per SIMD (62 VGPRs).  A tuning alternative to
-// k_sweep_packed (cse_set_sweep_kernel); the results are identical.
+// variables, an aligned batch as eight groups of 8 terms.  Eight waves per
+// SIMD (64 VGPRs).  The engine's choice for sampled calls of |H^t> terms
+// only, else a tuning alternative to k_sweep_packed (cse_set_sweep_kernel);
+// the results are identical.
 extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed_gen(SweepArgs A) {
     sweep_body<true, true, 0>(A);
 }

@@ -184,9 +184,10 @@ class Engine:
 
     def set_sweep_kernel(self, kernel, mode=0):
         """sweep kernel of one mode (cse_set_sweep_kernel): sampled (mode 0):
-        0 the engine's choice, 1 the eight-wave kernel with the two-column
-        asm path, 2 the general one; exact (mode 1): 0 the engine's choice,
-        1 seven waves per SIMD, 2 eight"""
+        0 the engine's choice (2 for calls of |H^t> instances only, else 1),
+        1 the eight-wave kernel with the two-column asm path, 2 the general
+        one; exact (mode 1): 0 the engine's choice (1), 1 seven waves per
+        SIMD, 2 eight"""
         self._check(self._L.cse_set_sweep_kernel(self.h, int(mode), int(kernel)))
         self.sweep_kernels[int(mode)] = int(kernel)
 

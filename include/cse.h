@@ -80,11 +80,12 @@ int cse_set_graphs(cse_ctx *ctx, int enable);
 /*
  * The sweep kernel of the context's calls of one mode, a tuning knob (the
  * results are identical).  mode 0 (sampled calls): kernel 0 the engine's
- * choice (1), 1 the eight-wave sweep whose hand-scheduled path takes
- * records of at most two non-pivot columns (others run compiled code), 2
- * the eight-wave sweep whose hand-scheduled path takes any number of
- * non-pivot columns and, for |H^t> terms, retires uncoupled variables in
- * one batch.  mode 1 (exact calls): kernel 0 the engine's choice (1), 1 the
+ * choice (2 when every instance of the call uses the |H^t> decomposition,
+ * else 1), 1 the eight-wave sweep whose hand-scheduled path takes records
+ * of at most two non-pivot columns (others run compiled code), 2 the
+ * eight-wave sweep whose hand-scheduled path takes any number of non-pivot
+ * columns and, for |H^t> terms, retires uncoupled variables in one batch
+ * and runs each aligned batch of 64 terms as eight groups of 8.  mode 1 (exact calls): kernel 0 the engine's choice (1), 1 the
  * exact sweep at seven waves per SIMD, 2 the same at eight.  No reference
  * counterpart (the reference has one CPU code path).
  */

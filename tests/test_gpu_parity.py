@@ -228,6 +228,58 @@ def test_exact_H_terms_when_a_shared_constraint_vanishes(E):
     assert np.count_nonzero(terms) > 0 and np.count_nonzero(terms == 0) > 0
 
 
+@pytest.mark.parametrize("kern", [1, 2])
+def test_exact_H_terms_when_a_group_constraint_vanishes(E, kern):
+    # -Z_8 Z_9 fixes x_8 + x_9 = 1 at t=16: pair 4 is bit 3 of j, which a
+    # group of 8 terms in an aligned batch shares, so that group's shared
+    # constraint vanishes while the other groups of its batch do not
+    t = 16
+    print(f"exact |H^t> group constraint t={t} kernel={kern}", flush=True)
+    E.set_sweep_kernel(kern, mode=1)
+    try:
+        terms = _exact_instance_terms(E, t, [2, 0], [0, 0b11], [0b11 << 8, 0], 0)
+    finally:
+        E.set_sweep_kernel(0, mode=1)
+    assert np.count_nonzero(terms) > 0 and np.count_nonzero(terms == 0) > 0
+
+
+@pytest.mark.parametrize("kern,t,r", [(1, 17, 6), (2, 17, 6), (1, 18, 9), (2, 18, 9)])
+def test_exact_H_terms_each_exact_kernel(E, kern, t, r):
+    # both exact-mode sweep kernels run aligned batches as eight groups of
+    # 8 terms, each from its own shared state
+    rng = np.random.default_rng(1700 + 10 * t + kern)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    print(f"exact |H^t> terms t={t} r={r} kernel={kern}", flush=True)
+    E.set_sweep_kernel(kern, mode=1)
+    try:
+        _exact_instance_terms(E, t, ph, xs, zs, 0)
+    finally:
+        E.set_sweep_kernel(0, mode=1)
+
+
+@pytest.mark.parametrize("kern", [1, 2])
+def test_sampled_H_terms_aligned_batches(E, kern):
+    # sampled |H^t> terms in items of 128 j: aligned 64-term batches, run as
+    # one batch (two-column kernel) or as eight groups (general kernel)
+    from circuitsimulator_amd.engine import Projector
+    t, r, k = 16, 6, 0
+    rng = np.random.default_rng(4242 + kern)
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    chi = 1 << ((t + 1) // 2)
+    seed, first, count = 77, 3, 24
+    print(f"sampled |H^t> terms t={t} r={r} kernel={kern}", flush=True)
+    E.set_sweep_kernel(kern)
+    try:
+        res, ex = E.sampled_projector(Projector.make(ph, xs, zs, t), [], k, 1, seed, first, count, chunk=128,
+                                      want_terms=True)
+    finally:
+        E.set_sweep_kernel(0)
+    se, zsum, terms = oracle_sampled(t, k, [], 1, ph, xs, zs, seed, first, count, chi)
+    assert np.array_equal(ex["sexp"], se)
+    assert np.array_equal(ex["terms"], terms)
+    assert np.array_equal(ex["zsum"], zsum)
+
+
 def test_batch_uneven_instances(E):
     from circuitsimulator_amd.engine import Instance, Projector
     rng = np.random.default_rng(2024)

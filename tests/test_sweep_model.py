@@ -882,6 +882,25 @@ def batch_ip_H_from(state, j, vary, t, count, iso=True):
     return W if s56 == END else 0
 
 
+def extend_H(state, jg, bits, t, count):
+    """A group's shared pair operations on top of its batch's state: those of
+    the bits of jg in `bits` (the group's shared bits that the batch's terms
+    do not share), as phiH_prefix applies them; None when the batch's state
+    already vanished or a group constraint vanishes."""
+    if state is None:
+        return None
+    smask, pairs = h_masks(t)
+    J0, G0, h, act, W = state
+    J, G = J0[:], G0[:]
+    cbits = jg & smask & bits
+    W = (W - 128 * popc(cbits)) & U32
+    st = h_ops(J, G, h, act, W, cbits, pairs & ~jg & bits, t, count)
+    if st is None:
+        return None
+    h, act, W = st
+    return J, G, h, act, W
+
+
 def batch_ip_H(r, j, t, count, iso=True):
     """One |H^t> term of phiH_batch with every pair operation its own: W
     starts at 128 (k + t - #constraints - n0) + 1 with Q in m.  Returns the
@@ -930,6 +949,34 @@ def test_phiH_batch_prefix_matches_oracle(t, r, seed):
     assert shared > 0 and count["h_restrict"] > 0 and count["h_couple"] > 0
 
 
+@pytest.mark.parametrize("t,r,seed", [(13, 6, 91), (16, 7, 92), (19, 5, 93), (22, 9, 94)])
+def test_phiH_two_level_prefix_matches_oracle(t, r, seed):
+    # an aligned batch of 64: the bits above the six low ones once per
+    # batch, bits 3..5 once per group of 8, bits 0..2 per term
+    from circuitsimulator_amd.workloads import commuting_paulis
+    rng = np.random.default_rng(seed)
+    ph, xs, zs = commuting_paulis(rng, t, r)
+    Po = orc.make_projector(ph, xs, zs, t)
+    chi = 1 << ((t + 1) // 2)
+    thr = orc.thresholds(t)
+    count = collections.Counter()
+    states = [orc.prepH(int(i), t) for i in rng.integers(0, chi, size=2)]
+    states += [orc.random_state(t, seed, s, thr) for s in range(2)]
+    for th in states:
+        if orc.project(th, Po) < 0:
+            continue
+        rec = to_record(th)
+        jb = 64 * int(rng.integers(0, chi // 64))
+        st = prefix_H(rec, jb, 63, t, count)
+        for g in range(8):
+            sg = extend_H(st, jb + 8 * g, 0x38, t, count)
+            for j in range(jb + 8 * g, jb + 8 * g + 8):
+                e, p, m = orc.inner(th, orc.prepH(j, t))
+                want = (e, p, m % 8) if e else (0, 0, 0)
+                assert decode(unpack(batch_ip_H_from(sg, j, 7, t, count))) == want, (t, jb, g, j)
+    assert count["h_restrict"] > 0 and count["h_couple"] > 0
+
+
 def test_phiH_batch_prefix_can_vanish():
     # a projector holding -Z_0 Z_1 fixes x_0 + x_1 = 1, so every j whose
     # highest pair bit (pair 0) is set gives zero: in an aligned batch that
@@ -952,6 +999,32 @@ def test_phiH_batch_prefix_can_vanish():
                 e, p, m = orc.inner(th, orc.prepH(j, t))
                 want = (e, p, m % 8) if e else (0, 0, 0)
                 assert decode(unpack(batch_ip_H_from(st, j, 63, t, count))) == want, (i, j)
+    assert vanished > 0
+
+
+def test_phiH_group_prefix_can_vanish():
+    # -Z_8 Z_9 fixes x_8 + x_9 = 1: at t = 16 pair 4 is bit 3 of j, shared by
+    # a group of 8 but not by the whole batch, so groups with bit 3 set
+    # vanish in their group prefix
+    t = 16
+    chi = 1 << ((t + 1) // 2)
+    Po = orc.make_projector([2, 0], [0, 0b11], [0b11 << 8, 0], t)
+    count = collections.Counter()
+    vanished = 0
+    for i in (5, 90, 201):
+        th = orc.prepH(i, t)
+        if orc.project(th, Po) < 0:
+            continue
+        rec = to_record(th)
+        for jb in range(0, chi, 64):
+            st = prefix_H(rec, jb, 63, t, count)
+            for g in range(8):
+                sg = extend_H(st, jb + 8 * g, 0x38, t, count)
+                vanished += sg is None
+                for j in range(jb + 8 * g, jb + 8 * g + 8, 3):
+                    e, p, m = orc.inner(th, orc.prepH(j, t))
+                    want = (e, p, m % 8) if e else (0, 0, 0)
+                    assert decode(unpack(batch_ip_H_from(sg, j, 7, t, count))) == want, (i, j)
     assert vanished > 0
 
 
