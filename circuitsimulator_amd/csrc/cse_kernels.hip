@@ -16,8 +16,14 @@
 //                    |L> terms of records with at most two non-pivot columns
 //                    run 64 at a time in one asm block (phiL_batch), every
 //                    |H^t> term in phiH_batch
-//   k_sweep_packed_all  the packed sweep of exact mode: every |L> term in
-//                    the asm block, any number of restrictions
+//   k_sweep_packed_gen  the sampled packed sweep with exact mode's general
+//                    restriction loop; aligned batches of |H^t> terms run as
+//                    eight groups of 8 that share their pair operations
+//   k_sweep_packed_all, k_sweep_packed_all8  the packed sweep of exact mode
+//                    (seven / eight waves per SIMD): every |L> term in the
+//                    asm block, any number of restrictions, every |H^t> term
+//                    after its batch's and group's shared pair operations
+//   k_zero_list      clears a call's result and queue blocks in one launch
 //   k_lspace_hist    histogram of |x| over the row space of L (decompose's
 //                    fidelity)
 #include "cse_device.h"
@@ -1648,9 +1654,10 @@ extern "C" __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) { sweep_b
 
 // the same sweep with D packed into the J rows (every instance has n <= 60;
 // row 60 is the elimination loop's end row).
-// Eight waves per SIMD (63 VGPRs, 78 SGPRs, one 8-byte spill stored at entry
-// and reloaded at exit): the elimination loop is bound by instruction issue,
-// and the extra wave hides more of its VALU -> SALU dependency latency.
+// Eight waves per SIMD (at most 64 VGPRs; the SGPRs beyond the budget spill
+// to VGPR lanes, nothing to scratch): the elimination loop is bound by
+// instruction issue, and the extra wave hides more of its VALU -> SALU
+// dependency latency.
 extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed(SweepArgs A) {
     sweep_body<true, false, 0>(A);
 }
@@ -1669,13 +1676,16 @@ extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed_gen(SweepArg
 // The packed sweep of exact mode: exact-mode records (projected |L> states of dimension about |x_i|) have
 // about n/2 non-pivot columns, so every |L> term runs in the asm block with
 // its general restriction loop, and every |H^t> term in phiH_batch.  The working G
-// rows take two more VGPRs: seven waves per SIMD, without spills.
+// rows take two more VGPRs: seven waves per SIMD (72 VGPRs), with the
+// persistent partial sums and queue state spilled to scratch around the asm
+// blocks, once per batch of 64 terms.
 extern "C" __global__ void __launch_bounds__(256, CSE_EXACT_BLOCKS_PER_CU) k_sweep_packed_all(SweepArgs A) {
     sweep_body<true, true, 1>(A);
 }
 
-// The same exact-mode sweep at eight waves per SIMD (64 VGPRs, more SGPRs
-// spilled to VGPR lanes): more latency hidden, more spill traffic.  A tuning
+// The same exact-mode sweep at eight waves per SIMD (64 VGPRs, more values
+// spilled to VGPR lanes and scratch): more latency hidden, more spill
+// traffic.  A tuning
 // alternative (cse_set_sweep_kernel); large mixed launches gain a few
 // percent, launches of many small instances lose a few.
 extern "C" __global__ void __launch_bounds__(256, 8) k_sweep_packed_all8(SweepArgs A) {

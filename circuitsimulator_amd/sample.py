@@ -18,6 +18,11 @@ exact defaults to False here, and the file option is disabled.  config may
 carry rng (a numpy Generator) for the uniform draws; numpy's global
 generator is used otherwise, as in the reference.
 """
+import numpy as np
+
+from .compile import gadgetize as _gadgetize
+from .compile import projectors as _projectors
+from .probability import decompose, probability
 
 
 def _next_psofar(chain, P, P0, Psofar, bit):
@@ -27,11 +32,6 @@ def _next_psofar(chain, P, P0, Psofar, bit):
     if chain != "reference":
         raise ValueError("chain must be 'reference' or 'joint'")
     return np.where(bit == 0, P0, 1 - P0)
-import numpy as np
-
-from .compile import gadgetize as _gadgetize
-from .compile import projectors as _projectors
-from .probability import decompose, probability
 
 
 def sampleQubits(circ, measure, sample, config={}):

@@ -167,7 +167,7 @@ def test_backend_sampled_mode_matches_engine():
     # the backend's numerator equals the engine's estimator on the same draws
     from circuitsimulator_amd import Engine
     from circuitsimulator_amd.engine import Projector
-    from circuitsimulator_amd.workloads import commuting_paulis, random_L
+    from circuitsimulator_amd.workloads import commuting_paulis
     rng = np.random.default_rng(9)
     t, k, r = 12, 4, 5
     ph, xs, zs = commuting_paulis(rng, t, r)

@@ -3,7 +3,6 @@
 for every inner product, full states where the device mirrors the reference's
 row bookkeeping, exact per-sample sums and histograms."""
 import ctypes
-import math
 from fractions import Fraction
 
 import numpy as np
