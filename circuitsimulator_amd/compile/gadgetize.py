@@ -133,24 +133,63 @@ def gadgetize_masks(circuit, Mdict, y):
     """gadgetize as (phases, xs, zs, size) with xs[g] (zs[g]) an int whose
     bit q is generator g's x (z) on qubit q."""
     pl, ng, size = _conjugate(circuit, Mdict, y)
+    xs, zs = _mask_rows(pl, ng, size)
     phases = [((pl.P0 >> g) & 1) + 2 * ((pl.P1 >> g) & 1) for g in range(ng)]
-    xs, zs = [0] * ng, [0] * ng
-    for q in range(size):
-        for planes, rows in ((pl.X, xs), (pl.Z, zs)):
-            m = planes[q]
-            while m:
-                g = (m & -m).bit_length() - 1
-                rows[g] |= 1 << q
-                m &= m - 1
     return phases, xs, zs, size
 
 
-def _conjugate(circuit, Mdict, y):
+def gadgetize_masks_pair(circuit, MG, MH, y):
+    """(gadgetize_masks(circuit, MG, y), gadgetize_masks(circuit, MH, y))
+    from one backward pass, when MG and MH agree on every measurement that
+    controls a gate (else two passes).  The pass conjugates Z_q once for
+    each qubit q of MG or MH and each ancilla; an outcome only flips the
+    generator's phase bit P1, which no gate reads, so each projector takes
+    its rows with its own outcomes applied at the end."""
+    if any(MG.get(m) != MH.get(m) for m in _controls(circuit)):
+        return gadgetize_masks(circuit, MG, y), gadgetize_masks(circuit, MH, y)
+    qs = list(MG.keys()) + [q for q in MH.keys() if q not in MG]
+    pl, ng, size = _conjugate(circuit, MG, y, qubits=qs)
+    xs, zs = _mask_rows(pl, ng, size)
+    slot = {q: i for i, q in enumerate(qs)}
+    anc = list(range(len(qs), ng))
+    out = []
+    for M in (MG, MH):
+        rows = [slot[q] for q in M.keys()] + anc
+        P1 = pl.P1 ^ sum(1 << slot[q] for q in M.keys() if M[q] == 1)
+        phases = [((pl.P0 >> g) & 1) + 2 * ((P1 >> g) & 1) for g in rows]
+        out.append((phases, [xs[g] for g in rows], [zs[g] for g in rows], size))
+    return out[0], out[1]
+
+
+@functools.lru_cache(maxsize=16)
+def _controls(circuit):
+    """The measurements that control a gate of circuit."""
+    return frozenset(m for m, _, _ in _steps(circuit) if m >= 0)
+
+
+def _mask_rows(pl, ng, size):
+    """(xs, zs): generator g's x (z) row as an int, bit q for qubit q."""
+    xs, zs = [0] * ng, [0] * ng
+    for q in range(size):
+        for planes, acc in ((pl.X, xs), (pl.Z, zs)):
+            m = planes[q]
+            while m:
+                g = (m & -m).bit_length() - 1
+                acc[g] |= 1 << q
+                m &= m - 1
+    return xs, zs
+
+
+def _conjugate(circuit, Mdict, y, qubits=None):
     """The bitplanes of gadgetize's generators after the backward pass, their
-    number and the qubit count n + len(y)."""
+    number and the qubit count n + len(y).  The generators are Z_q with
+    outcome Mdict[q] for q in Mdict (or Z_q with outcome 0 for q in
+    `qubits`, when given), then Z on each ancilla with outcome y; Mdict
+    decides the classically controlled gates either way."""
     n = len(standardGate(circuit.split("\n", 1)[0], lineMode=True))
     size = n + len(y)
-    gens = [(q, Mdict[q]) for q in Mdict.keys()] + [(q, y[q - n]) for q in range(n, size)]
+    first = [(q, Mdict[q]) for q in Mdict.keys()] if qubits is None else [(q, 0) for q in qubits]
+    gens = first + [(q, y[q - n]) for q in range(n, size)]
     ng = len(gens)
     pl = _Planes(size)
     for g, (q, outcome) in enumerate(gens):

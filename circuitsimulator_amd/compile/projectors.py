@@ -37,8 +37,7 @@ def projectors_masks(circ, measure, verbose=False, x=None, y=None, rng=None):
     """projectors() with G and H as (phases, xs, zs, n + t), each x and z row
     an int (bit q for qubit q); the same draws and messages."""
     Mdict, y, n, t = _outcomes(circ, measure, verbose, x, y, rng)
-    G = gadgetize.gadgetize_masks(circ, measure, y)
-    H = gadgetize.gadgetize_masks(circ, Mdict, y)
+    G, H = gadgetize.gadgetize_masks_pair(circ, measure, Mdict, y)
     return G, H, n, t
 
 
@@ -126,7 +125,7 @@ def _null_space_masks(cols, r):
     for cm in cols:
         good, bad = [], []
         for v in X:
-            (bad if bin(v & cm).count("1") & 1 else good).append(v)
+            (bad if (v & cm).bit_count() & 1 else good).append(v)
         X = good + [bad[j] ^ bad[j - 1] for j in range(1, len(bad))]
     return X
 
@@ -170,19 +169,22 @@ def null_products(n, phases, xm, zm):
     the product of the generators in v, in index order as _multiply takes
     them, with x and z shifted down to the qubits from n on."""
     r = len(phases)
-    cols = []
-    for c in range(n):
-        cm = 0
-        for i in range(r):
-            cm |= ((xm[i] >> c) & 1) << i
-        cols.append(cm)
+    # cols[c]: the generators with an X on circuit qubit c
+    cols = [0] * n
+    low = (1 << n) - 1
+    for i in range(r):
+        m = xm[i] & low
+        while m:
+            c = (m & -m).bit_length() - 1
+            cols[c] |= 1 << i
+            m &= m - 1
     out = []
     for v in _null_space_masks(cols, r):
         ph, x, z = 0, 0, 0
         i, w = 0, v
         while w:
             if w & 1:
-                ph = (ph + int(phases[i]) + 2 * bin(z & xm[i]).count("1")) % 4
+                ph = (ph + int(phases[i]) + 2 * (z & xm[i]).bit_count()) % 4
                 x ^= xm[i]
                 z ^= zm[i]
             w >>= 1

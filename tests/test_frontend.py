@@ -338,6 +338,39 @@ def test_gadgetize_bitplanes_match_per_generator_tables():
                 assert xs[g].dtype == np.int64 and zs[g].dtype == np.int64
 
 
+def test_gadgetize_pair_matches_two_passes(libdir):
+    # G and H from one backward pass (gadgetize_masks_pair) equal two passes,
+    # on circuits with measurements and classically controlled gates, with
+    # requested outcomes that agree with the circuit's (one pass) and that
+    # contradict a controlling one (two passes)
+    rng = np.random.default_rng(78)
+    circs = ["_H\nCX\nHM\nH_\nTM\nX_\n", compile_main(libdir, "H_\nCS\nV_\n_H\n_Q\nCZ\nH_\n")]
+    for _ in range(8):
+        n = int(rng.integers(2, 6))
+        while True:
+            c = random_clifford_t(rng, n, int(rng.integers(10, 30)), int(rng.integers(0, 5)))
+            if valid_ts(c):
+                break
+        circs.append(c)
+    seen_fallback = False
+    for circ in circs:
+        t, Ms, MTs = gadgetize.countY(circ)
+        n = len(circ.splitlines()[0])
+        for trial in range(4):
+            MH = {q: int(rng.integers(0, 2)) for q in Ms}
+            qs = [q for q in range(n) if q not in Ms]
+            MG = {int(q): int(rng.integers(0, 2)) for q in rng.permutation(qs)[: int(rng.integers(0, len(qs) + 1))]}
+            for q in Ms:
+                MG[q] = MH[q] if trial < 3 else 1 - MH[q]
+            seen_fallback |= trial == 3 and bool(gadgetize._controls(circ))
+            nt = t + sum(1 for q in MTs if MH[q] == 1)
+            y = [int(b) for b in rng.integers(0, 2, size=nt)]
+            got = gadgetize.gadgetize_masks_pair(circ, MG, MH, y)
+            want = (gadgetize.gadgetize_masks(circ, MG, y), gadgetize.gadgetize_masks(circ, MH, y))
+            assert got == want, (circ, MG, MH, y)
+    assert seen_fallback
+
+
 def valid_ts(circ):
     try:
         gadgetize.countY(circ)
