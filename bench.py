@@ -572,7 +572,7 @@ def main():
                     help="sweep kernel of the configuration's mode (cse_set_sweep_kernel; 0: the engine's choice)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"c1": 200, "c4": 20}.get(args.config, 3)
+        args.steps = {"c1": 1000, "c4": 20}.get(args.config, 3)
     if args.warmup is None:
         args.warmup = 20 if args.config == "c1" else 1
     world = int(os.environ.get("WORLD_SIZE", "1"))
