@@ -547,7 +547,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 3; c1: 200 probability evaluations of about 0.5 ms each)")
+                    help="timed steps (default 3; c1: 1,000 probability evaluations of about 0.13 ms each; c4: 20)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1; c1: 20)")
     ap.add_argument("--samples", type=int, default=65536, help="random thetas per step (M)")
     ap.add_argument("--chunk", default=None,
