@@ -256,13 +256,14 @@ def test_exact_H_terms_each_exact_kernel(E, kern, t, r):
         E.set_sweep_kernel(0, mode=1)
 
 
-@pytest.mark.parametrize("kern", [1, 2])
-def test_sampled_H_terms_aligned_batches(E, kern):
+@pytest.mark.parametrize("kern,t", [(1, 16), (2, 16), (2, 17)])
+def test_sampled_H_terms_aligned_batches(E, kern, t):
     # sampled |H^t> terms in items of 128 j: aligned 64-term batches, run as
-    # one batch (two-column kernel) or as eight groups (general kernel)
+    # one batch (two-column kernel) or as eight groups (general kernel); odd
+    # t has a last "pair" of one qubit (a constraint, never a coupling)
     from circuitsimulator_amd.engine import Projector
-    t, r, k = 16, 6, 0
-    rng = np.random.default_rng(4242 + kern)
+    r, k = 6, 0
+    rng = np.random.default_rng(4242 + kern + 100 * t)
     ph, xs, zs = random_commuting_projector(rng, t, r)
     chi = 1 << ((t + 1) // 2)
     seed, first, count = 77, 3, 24
