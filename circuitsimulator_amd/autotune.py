@@ -3,9 +3,9 @@ work is cut and which kernel runs it, never the results.
 
 Exact mode (cse_batch_exact): the sweep's occupancy and the claim cap.
 Kernel 1 runs the exact sweep at seven waves per SIMD, kernel 2 at eight
-(more latency hidden, more SGPRs spilled to VGPR lanes): large mixed
-launches gain a few percent at eight, launches of many small instances lose
-a few.  The sweep hands out slices of 64 j, a static first claim per wave
+(more latency hidden, more values spilled): long launches gain a few percent
+at eight, short ones (a few slices per wave) lose a few, and the engine's
+rule (kernel 0) takes eight from 16 slices per resident wave.  The sweep hands out slices of 64 j, a static first claim per wave
 and then guided self-scheduling from eight counters; `chunk` caps a claim.
 Larger caps cut the per-segment cost (the record load, the lane tables) on
 launches with many slices per wave; the guided claims shrink at the end of

@@ -483,7 +483,12 @@ int prep_batch(cse_ctx *ctx, const Plan &pl, int mode) {
 enum SweepKernel { SK_GENERIC, SK_PACKED, SK_PACKED_GEN, SK_ALL, SK_ALL8 };
 
 // The sweep kernel of a launch.  Exact mode takes k_sweep_packed_all (seven
-// waves per SIMD) or, when set, k_sweep_packed_all8.  Sampled launches take
+// waves per SIMD) or k_sweep_packed_all8 (eight) as set, and by the engine's
+// rule the eight-wave kernel when the launch holds at least
+// CSE_ALL8_SLICES_PER_WAVE slices for each of its waves: a long launch gains
+// from the extra wave's latency hiding (the uneven batch 2.5%, the t=30 |H^t>
+// projector 2.6%), a short one loses more to its spills than it gains (one
+// projector of 2,048 records, 4.6 slices per wave: 4.5%).  Sampled launches take
 // k_sweep_packed (eight waves, the two-column asm path for |L> terms), or
 // k_sweep_packed_gen when set or when every term is an |H^t> term: that
 // kernel runs an aligned batch of |H^t> terms as eight groups of 8 that share
@@ -495,7 +500,11 @@ SweepKernel sweep_choice(const cse_ctx *ctx, const Plan &pl, int mode) {
         allH = allH && pl.inst[i].exact;
     }
     if (!packed) return SK_GENERIC;
-    if (mode == 1) return ctx->sweep_kernel[1] == 2 ? SK_ALL8 : SK_ALL;
+    if (mode == 1) {
+        const int k = ctx->sweep_kernel[1];
+        if (k) return k == 2 ? SK_ALL8 : SK_ALL;
+        return pl.nitems >= CSE_ALL8_SLICES_PER_WAVE * 32ll * ctx->cus ? SK_ALL8 : SK_ALL;
+    }
     const int k = ctx->sweep_kernel[0];
     return k == 2 || (k == 0 && allH) ? SK_PACKED_GEN : SK_PACKED;
 }

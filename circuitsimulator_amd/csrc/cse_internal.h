@@ -59,6 +59,11 @@ inline __host__ __device__ long long tri_items(long long chi, long long ch, long
 #ifndef CSE_GSS_DIV
 #define CSE_GSS_DIV 2
 #endif
+// The engine's exact-mode rule takes the eight-wave sweep for launches of at
+// least this many 64-j slices per resident wave (32 waves per CU)
+#ifndef CSE_ALL8_SLICES_PER_WAVE
+#define CSE_ALL8_SLICES_PER_WAVE 16
+#endif
 #ifndef CSE_STATIC_DIV
 #define CSE_STATIC_DIV 2
 #endif

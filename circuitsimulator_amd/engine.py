@@ -186,8 +186,9 @@ class Engine:
         """sweep kernel of one mode (cse_set_sweep_kernel): sampled (mode 0):
         0 the engine's choice (2 for calls of |H^t> instances only, else 1),
         1 the eight-wave kernel with the two-column asm path, 2 the general
-        one; exact (mode 1): 0 the engine's choice (1), 1 seven waves per
-        SIMD, 2 eight"""
+        one; exact (mode 1): 0 the engine's choice (2 for launches of at
+        least 16 slices per resident wave, else 1), 1 seven waves per SIMD,
+        2 eight"""
         self._check(self._L.cse_set_sweep_kernel(self.h, int(mode), int(kernel)))
         self.sweep_kernels[int(mode)] = int(kernel)
 

@@ -85,9 +85,11 @@ int cse_set_graphs(cse_ctx *ctx, int enable);
  * of at most two non-pivot columns (others run compiled code), 2 the
  * eight-wave sweep whose hand-scheduled path takes any number of non-pivot
  * columns and, for |H^t> terms, retires uncoupled variables in one batch
- * and runs each aligned batch of 64 terms as eight groups of 8.  mode 1 (exact calls): kernel 0 the engine's choice (1), 1 the
- * exact sweep at seven waves per SIMD, 2 the same at eight.  No reference
- * counterpart (the reference has one CPU code path).
+ * and runs each aligned batch of 64 terms as eight groups of 8.  mode 1
+ * (exact calls): kernel 0 the engine's choice (2 for launches of at least 16
+ * slices of 64 j per resident wave, else 1), 1 the exact sweep at seven
+ * waves per SIMD, 2 the same at eight.  No reference counterpart (the
+ * reference has one CPU code path).
  */
 int cse_set_sweep_kernel(cse_ctx *ctx, int mode, int kernel);
 
