@@ -1014,6 +1014,158 @@ DEV unsigned uni32(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane
     "s_mov_b32 m0, s61\n\t" \
     "s_nop 1\n\t"
 
+// One term's own pair operation for bit B of j in a group of 8 (PHIH_GROUP):
+// a constraint when the bit is set, a coupling when it is clear and a pair
+// bit (of pmask), nothing otherwise; SH is 2 q for the bit's pair q.  The
+// same register use and arithmetic as PHIH_BODY's loops, with the pair's
+// masks shifted by a shift computed once per call instead of per operation.
+#define PHIH_OWN_BIT(B, SH) \
+    "s_bitcmp1_b32 s62, " #B "\n\t" \
+    "s_cbranch_scc0 L_gc" #B "%=\n\t" \
+    "s_lshl_b64 s[52:53], 3, " SH "\n\t" \
+    "s_and_b64 s[66:67], s[48:49], s[52:53]\n\t" \
+    "s_bcnt1_i32_b64 s51, s[66:67]\n\t" \
+    "v_and_b32 %[vt0], s52, %[glo]\n\t" \
+    "v_and_or_b32 %[vt0], s53, %[ghi], %[vt0]\n\t" \
+    "v_bcnt_u32_b32 %[vt0], %[vt0], 0\n\t" \
+    "v_and_b32 %[vt0], 1, %[vt0]\n\t" \
+    "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t" \
+    "s_and_b64 s[64:65], s[52:53], s[58:59]\n\t" \
+    "s_cbranch_scc0 L_ge" #B "%=\n\t" \
+    "s_ff1_i32_b64 s56, s[64:65]\n\t" \
+    "s_bitset0_b64 s[58:59], s56\n\t" \
+    "v_readlane_b32 s66, %[jlo], s56\n\t" \
+    "v_readlane_b32 s67, %[jhi], s56\n\t" \
+    "v_readlane_b32 s46, %[glo], s56\n\t" \
+    "v_readlane_b32 s47, %[ghi], s56\n\t" \
+    "s_and_b32 s56, s67, 0xe0000000\n\t" \
+    "s_and_b32 s67, s67, 0x1fffffff\n\t" \
+    "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t" \
+    "s_nop 1\n\t" \
+    "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t" \
+    "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t" \
+    "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t" \
+    "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t" \
+    "v_bitop3_b32 %[glo], %[glo], s46, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[ghi], %[ghi], s47, %[vt0] bitop3:0x78\n\t" \
+    "s_bitcmp1_b32 s56, 30\n\t" \
+    "s_cbranch_scc0 L_gnojii" #B "%=\n\t" \
+    "v_xor_b32 %[vt3], %[vt3], %[vt0]\n" \
+    "L_gnojii" #B "%=:\n\t" \
+    "s_add_u32 s60, s60, 0x80\n\t" \
+    "s_bitcmp1_b32 s51, 0\n\t" \
+    "s_cbranch_scc0 L_gnoc" #B "%=\n\t" \
+    "s_or_b32 s65, s65, 0x80000000\n\t" \
+    "s_add_u32 s60, s60, s56\n\t" \
+    "s_xor_b64 s[48:49], s[48:49], s[46:47]\n" \
+    "L_gnoc" #B "%=:\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t" \
+    "s_branch L_gn" #B "%=\n" \
+    "L_ge" #B "%=:\n\t" \
+    "s_bitcmp1_b32 s51, 0\n\t" \
+    "s_cbranch_scc1 L_zero%=\n\t" \
+    "s_branch L_gn" #B "%=\n" \
+    "L_gc" #B "%=:\n\t" \
+    "s_bitcmp1_b32 %[pmask], " #B "\n\t" \
+    "s_cbranch_scc0 L_gn" #B "%=\n\t" \
+    "s_lshl_b64 s[52:53], 1, " SH "\n\t" \
+    "s_lshl_b64 s[46:47], 2, " SH "\n\t" \
+    "v_and_b32 %[vt0], s52, %[glo]\n\t" \
+    "v_and_or_b32 %[vt0], s53, %[ghi], %[vt0]\n\t" \
+    "v_and_b32 %[vt1], s46, %[glo]\n\t" \
+    "v_and_or_b32 %[vt1], s47, %[ghi], %[vt1]\n\t" \
+    "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t" \
+    "v_cmp_ne_u32_e64 s[46:47], 0, %[vt1]\n\t" \
+    "s_lshr_b64 s[66:67], s[48:49], " SH "\n\t" \
+    "s_and_b64 s[52:53], s[52:53], s[58:59]\n\t" \
+    "s_and_b64 s[46:47], s[46:47], s[58:59]\n\t" \
+    "s_and_b32 s67, s66, 3\n\t" \
+    "s_cmp_eq_u32 s67, 3\n\t" \
+    "s_cselect_b32 s67, 0x80000000, 0\n\t" \
+    "s_add_u32 s60, s60, s67\n\t" \
+    "s_bitcmp1_b32 s66, 0\n\t" \
+    "s_cselect_b64 s[64:65], s[46:47], 0\n\t" \
+    "s_bitcmp1_b32 s66, 1\n\t" \
+    "s_cselect_b64 s[66:67], s[52:53], 0\n\t" \
+    "s_xor_b64 s[64:65], s[64:65], s[66:67]\n\t" \
+    "s_and_b64 s[66:67], s[52:53], s[46:47]\n\t" \
+    "s_xor_b64 s[64:65], s[64:65], s[66:67]\n\t" \
+    "v_cndmask_b32_e64 %[vt0], 0, -1, s[52:53]\n\t" \
+    "v_cndmask_b32_e64 %[vt1], 0, -1, s[46:47]\n\t" \
+    "v_cndmask_b32_e64 %[vt2], 0, -1, s[64:65]\n\t" \
+    "v_and_b32 %[vt2], 0x80000000, %[vt2]\n\t" \
+    "v_xor_b32 %[jhi], %[jhi], %[vt2]\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s46, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s47, %[vt0] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jlo], %[jlo], s52, %[vt1] bitop3:0x78\n\t" \
+    "v_bitop3_b32 %[jhi], %[jhi], s53, %[vt1] bitop3:0x78\n" \
+    "L_gn" #B "%=:\n\t"
+
+// PHIH_BODY for the terms of one group of 8 in an aligned batch: bits 3 and
+// up of j are the group's (applied by phiH_prefix), so a term's own
+// operations are those of bits 0..2, unrolled in bit order (the order of
+// pair operations does not change a term's value).  s62 holds the term's
+// set bits among them.
+#define PHIH_GROUP_BODY(LOOP) \
+    "v_mov_b32 v40, 0xc0000000\n\t" \
+    "v_mov_b32 v41, 0x80000000\n\t" \
+    "s_mov_b32 s61, m0\n\t" \
+    "s_mov_b32 m0, %[m0s]\n" \
+    "L_ip%=:\n\t" \
+    "s_add_u32 s46, %[jb], m0\n\t" \
+    "s_and_b32 s62, s46, %[smask]\n\t" \
+    "s_bcnt1_i32_b32 s47, s62\n\t" \
+    "s_lshl_b32 s47, s47, 7\n\t" \
+    "s_sub_u32 s60, %[wbase], s47\n\t" \
+    "v_mov_b32 %[jlo], %[jlo0]\n\t" \
+    "v_mov_b32 %[jhi], %[jhi0]\n\t" \
+    "v_mov_b32 %[glo], %[g0lo]\n\t" \
+    "v_mov_b32 %[ghi], %[g0hi]\n\t" \
+    "s_mov_b64 s[48:49], %[h0]\n\t" \
+    "s_mov_b64 s[58:59], %[actR]\n\t" \
+    PHIH_OWN_BIT(0, "%[sh0]") \
+    PHIH_OWN_BIT(1, "%[sh1]") \
+    PHIH_OWN_BIT(2, "%[sh2]") \
+    "L_elim%=:\n" \
+    LOOP \
+    "L_end%=:\n\t" \
+    "s_cmp_eq_u32 s56, 60\n\t" \
+    "s_cselect_b32 s60, s60, 0\n" \
+    "L_emit%=:\n\t" \
+    "v_writelane_b32 %[parked], s60, m0\n\t" \
+    "s_add_u32 m0, m0, 1\n\t" \
+    "s_cmp_lt_u32 m0, %[nb]\n\t" \
+    "s_cbranch_scc1 L_ip%=\n\t" \
+    "s_branch L_done%=\n" \
+    "L_zero%=:\n\t" \
+    "s_mov_b32 s60, 0\n\t" \
+    "s_branch L_emit%=\n" \
+    "L_done%=:\n\t" \
+    "s_mov_b32 m0, s61\n\t" \
+    "s_nop 1\n\t"
+
+// Terms jb + m0s .. nb - 1 (a group of 8 of an aligned batch, from the
+// group's shared state) into lanes m0s .. nb - 1 of parked, which keeps its
+// other lanes; smask and pmask restricted to bits 0..2, sh_b = 2 (sm1 - b).
+DEV int phiH_group(unsigned nb, unsigned jb, unsigned jlo0, unsigned jhi0, u64 g0, u64 actR, u64 h0,
+                   unsigned wbase, unsigned smask, unsigned pmask, unsigned sh0, unsigned sh1, unsigned sh2,
+                   unsigned m0s, int parked) {
+    unsigned jlo, jhi, vt0, vt1, vt2, vt3, glo, ghi;
+    asm volatile(
+        PHIH_GROUP_BODY(CSE_ELIM_LOOP_ISO)
+        : [parked] "+&v"(parked), [jlo] "=&v"(jlo), [jhi] "=&v"(jhi), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
+          [vt2] "=&v"(vt2), [vt3] "=&v"(vt3), [glo] "=&v"(glo), [ghi] "=&v"(ghi)
+        : [nb] "s"(nb), [jb] "s"(jb), [jlo0] "v"(jlo0), [jhi0] "v"(jhi0), [g0lo] "v"((unsigned)g0),
+          [g0hi] "v"((unsigned)(g0 >> 32)), [actR] "s"(actR), [h0] "s"(h0), [wbase] "s"(wbase), [smask] "s"(smask),
+          [pmask] "s"(pmask), [sh0] "s"(sh0), [sh1] "s"(sh1), [sh2] "s"(sh2), [m0s] "s"(m0s)
+        : "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s56", "s58", "s59",
+          "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "v40", "v41", "scc");
+    return parked;
+}
+
 // Terms jb + m0s .. jb + nb - 1 into lanes m0s .. nb - 1 of parked, which
 // keeps its other lanes (a batch can run as several calls, each from its own
 // shared state).
@@ -1453,10 +1605,11 @@ DEV void sweep_body(const SweepArgs &A) {
                             unsigned W8 = W;
                             const unsigned jg = (unsigned)jb + g;
                             if (phiH_prefix(jg & smask & 0x38u, pmask & ~jg & 0x38u, size - 1u, J8, G8, h8, act8, W8))
-                                parked = phiH_batch<ALL>(uni32(g + 8u), uni32((unsigned)jb), (unsigned)J8,
-                                                         (unsigned)(J8 >> 32), G8, uniform64(act8), uniform64(h8),
-                                                         uni32(W8), uni32(smask & 7u), uni32(size - 1u),
-                                                         uni32(pmask & 7u), uni32(g), parked);
+                                parked = phiH_group(uni32(g + 8u), uni32((unsigned)jb), (unsigned)J8,
+                                                    (unsigned)(J8 >> 32), G8, uniform64(act8), uniform64(h8),
+                                                    uni32(W8), uni32(smask & 7u), uni32(pmask & 7u),
+                                                    uni32(2u * (size - 1u)), uni32(2u * (size - 2u)),
+                                                    uni32(2u * (size - 3u)), uni32(g), parked);
                         }
                     } else {
                         parked = phiH_batch<ALL>(uni32((unsigned)nb), uni32((unsigned)jb), (unsigned)J,

@@ -901,6 +901,26 @@ def extend_H(state, jg, bits, t, count):
     return J, G, h, act, W
 
 
+def batch_ip_H_bits(state, j, t, count, iso=True):
+    """One term of a group of 8 (phiH_group: bits 0..2 of j its own), the
+    three bits in ascending order, each a constraint (set) or a coupling
+    (clear, a pair bit), then the loop; returns the parked W word."""
+    if state is None:
+        return 0
+    smask, pairs = h_masks(t)
+    J0, G0, h, act, W = state
+    J, G = J0[:], G0[:]
+    W = (W - 128 * popc(j & smask & 7)) & U32
+    for b in range(3):
+        bit_b = 1 << b
+        st = h_ops(J, G, h, act, W, j & smask & bit_b, pairs & ~j & bit_b, t, count)
+        if st is None:
+            return 0
+        h, act, W = st
+    s56, W = elim_loop(J, act, W, count, iso=True if iso else False)
+    return W if s56 == END else 0
+
+
 def batch_ip_H(r, j, t, count, iso=True):
     """One |H^t> term of phiH_batch with every pair operation its own: W
     starts at 128 (k + t - #constraints - n0) + 1 with Q in m.  Returns the
@@ -974,6 +994,8 @@ def test_phiH_two_level_prefix_matches_oracle(t, r, seed):
                 e, p, m = orc.inner(th, orc.prepH(j, t))
                 want = (e, p, m % 8) if e else (0, 0, 0)
                 assert decode(unpack(batch_ip_H_from(sg, j, 7, t, count))) == want, (t, jb, g, j)
+                # the same term with its three pair operations in bit order
+                assert decode(unpack(batch_ip_H_bits(sg, j, t, count))) == want, ("bits", t, jb, g, j)
     assert count["h_restrict"] > 0 and count["h_couple"] > 0
 
 
