@@ -130,10 +130,118 @@ DEV void as_subtract_general(AState &s, const RState &s2) {
     s.Q = (s.Q - Q2 + 8) & 7;
 }
 
-// <s2|s1> for general packed states (stabilizer.c:589-659)
+// The restrictions of innerProductExact (stabilizer.c:594-607) on the packed
+// layout (D_x in bits 61..63 of J_x, n <= 60), one asm block: for each row b
+// of Gbar2 from k2 to n - 1, xi = that row and the right side alpha =
+// h2 . xi; the column is the ballot of the parity of G & xi over the active
+// rows, c = (h2 ^ h) . xi, and the elimination of i = min(S) over S is the
+// exact-mode sweep's (PHIL_RESTRICT_ALL): rows of S take J_i, D_i + 4 J_xi
+// and G_i, the rows of M = J_i (^ S when D_i is in {2,6}) take S, and c = 1
+// adds D_i to Q (bits 29..31 of s60), 4 to D on M and G_i to h.  Returns 0
+// when a restriction is empty with right side 1 (the product is 0).
+DEV int pk_restrict_general(unsigned &jlo, unsigned &jhi, unsigned &glo, unsigned &ghi, u64 &h, u64 &act,
+                            unsigned &W, unsigned gblo, unsigned gbhi, u64 h2, unsigned bstart, unsigned bend) {
+    unsigned vt0, vt1, vt2, vt3;
+    int ok;
+    asm volatile(
+        "s_mov_b32 %[ok], 1\n\t"
+        "s_mov_b32 s50, %[bstart]\n"
+        "L_gb%=:\n\t"
+        "s_cmp_ge_u32 s50, %[bend]\n\t"
+        "s_cbranch_scc1 L_gdone%=\n\t"
+        "v_readlane_b32 s52, %[gblo], s50\n\t"
+        "v_readlane_b32 s53, %[gbhi], s50\n\t"
+        "s_add_u32 s50, s50, 1\n\t"
+        "s_xor_b64 s[66:67], %[h2], s[48:49]\n\t"
+        "s_and_b64 s[66:67], s[66:67], s[52:53]\n\t"
+        "s_bcnt1_i32_b64 s51, s[66:67]\n\t"
+        "v_and_b32 %[vt0], s52, %[glo]\n\t"
+        "v_and_b32 %[vt1], s53, %[ghi]\n\t"
+        "v_bcnt_u32_b32 %[vt0], %[vt0], 0\n\t"
+        "v_bcnt_u32_b32 %[vt0], %[vt1], %[vt0]\n\t"
+        "v_and_b32 %[vt0], 1, %[vt0]\n\t"
+        "v_cmp_ne_u32_e64 s[52:53], 0, %[vt0]\n\t"
+        "s_and_b64 s[64:65], s[52:53], s[58:59]\n\t"
+        "s_cbranch_scc0 L_gempty%=\n\t"
+        "s_ff1_i32_b64 s56, s[64:65]\n\t"
+        "s_bitset0_b64 s[58:59], s56\n\t"
+        "v_readlane_b32 s66, %[jlo], s56\n\t"
+        "v_readlane_b32 s67, %[jhi], s56\n\t"
+        "v_readlane_b32 s46, %[glo], s56\n\t"
+        "v_readlane_b32 s47, %[ghi], s56\n\t"
+        "s_and_b32 s56, s67, 0xe0000000\n\t"
+        "s_and_b32 s67, s67, 0x1fffffff\n\t"
+        "v_cndmask_b32_e64 %[vt0], 0, -1, s[64:65]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[vt3], 0, -1, s[66:67]\n\t"
+        "v_bitop3_b32 %[jlo], %[jlo], s66, %[vt0] bitop3:0x78\n\t"
+        "v_bitop3_b32 %[jhi], %[jhi], s67, %[vt0] bitop3:0x78\n\t"
+        "v_and_b32 %[vt1], 0x80000000, %[vt3]\n\t"
+        "v_bitop3_b32 %[vt2], %[vt0], s56, %[vt1] bitop3:0x60\n\t"
+        "v_add_u32 %[jhi], %[jhi], %[vt2]\n\t"
+        "v_bitop3_b32 %[glo], %[glo], s46, %[vt0] bitop3:0x78\n\t"
+        "v_bitop3_b32 %[ghi], %[ghi], s47, %[vt0] bitop3:0x78\n\t"
+        "s_bitcmp1_b32 s56, 30\n\t"
+        "s_cbranch_scc0 L_gnojii%=\n\t"
+        "v_xor_b32 %[vt3], %[vt3], %[vt0]\n"
+        "L_gnojii%=:\n\t"
+        "s_bitcmp1_b32 s51, 0\n\t"
+        "s_cbranch_scc0 L_gnoc%=\n\t"
+        "s_or_b32 s65, s65, 0x80000000\n\t"
+        "s_add_u32 s60, s60, s56\n\t"
+        "s_xor_b64 s[48:49], s[48:49], s[46:47]\n"
+        "L_gnoc%=:\n\t"
+        "v_bitop3_b32 %[jlo], %[jlo], s64, %[vt3] bitop3:0x78\n\t"
+        "v_bitop3_b32 %[jhi], %[jhi], s65, %[vt3] bitop3:0x78\n\t"
+        "s_branch L_gb%=\n"
+        "L_gempty%=:\n\t"
+        "s_bitcmp1_b32 s51, 0\n\t"
+        "s_cbranch_scc0 L_gb%=\n\t"
+        "s_mov_b32 %[ok], 0\n"
+        "L_gdone%=:\n\t"
+        : [ok] "=&s"(ok), [jlo] "+v"(jlo), [jhi] "+v"(jhi), [glo] "+v"(glo), [ghi] "+v"(ghi),
+          "+{s[48:49]}"(h), "+{s[58:59]}"(act), "+{s60}"(W), [vt0] "=&v"(vt0), [vt1] "=&v"(vt1),
+          [vt2] "=&v"(vt2), [vt3] "=&v"(vt3)
+        : [gblo] "v"(gblo), [gbhi] "v"(gbhi), [h2] "s"(h2), [bstart] "s"(bstart), [bend] "s"(bend)
+        : "s46", "s47", "s50", "s51", "s52", "s53", "s56", "s64", "s65", "s66", "s67", "scc");
+    return ok;
+}
+
+// <s2|s1> for general packed states (stabilizer.c:589-659).  For n <= 60
+// the restrictions run in pk_restrict_general and the exponential sum in
+// the hand-scheduled loop (as_expsum_packed), on the packed layout.
 DEV void ip_general(const RState &r1, const RState &r2, int &eps, int &p, int &m) {
     AState s;
     to_astate(r1, s);
+    if (r1.n <= CSE_END_ROW) {
+        const int x = lane_id();
+        unsigned jlo = (unsigned)s.J, jhi = (unsigned)(s.J >> 32) | ((unsigned)s.d << 29);
+        unsigned glo = (unsigned)s.G, ghi = (unsigned)(s.G >> 32);
+        u64 h = uniform64(s.h), act = uniform64(s.act);
+        unsigned W = (unsigned)__builtin_amdgcn_readfirstlane((int)((unsigned)s.Q << 29));
+        const int ok = pk_restrict_general(jlo, jhi, glo, ghi, h, act, W, (unsigned)r2.Gb, (unsigned)(r2.Gb >> 32),
+                                           uniform64(r2.h), (unsigned)__builtin_amdgcn_readfirstlane(r2.k),
+                                           (unsigned)__builtin_amdgcn_readfirstlane(r2.n));
+        if (!ok) {
+            eps = 0; p = 0; m = 0;
+            return;
+        }
+        s.J = ((u64)(jhi & 0x1fffffffu) << 32) | jlo;
+        s.d = (int)(jhi >> 29);
+        s.G = ((u64)ghi << 32) | glo;
+        s.h = h;
+        s.act = act;
+        s.Q = (int)(W >> 29);
+        as_subtract_general(s, r2);
+        // packed again for the loop: D in bits 61..63, row 60 the end row
+        s.J = (s.J & lowmask(CSE_END_ROW)) | ((u64)(s.d & 7) << 61);
+        if (x == CSE_END_ROW) s.J = 4ull << 61;
+        int pp, mm;
+        eps = as_expsum_packed(s, pp, mm);
+        p = eps ? pp - r1.k - r2.k : 0;
+        m = mm;
+        return;
+    }
     for (int b = r2.k; b < r2.n; b++) {
         u64 xi = rl64(r2.Gb, b);
         int alpha = par64(r2.h & xi);
