@@ -42,8 +42,10 @@ def _mixed_states(n, seed, count=10):
     return states
 
 
-@pytest.mark.parametrize("n", [1, 3, 8, 24, 48, 64])
+@pytest.mark.parametrize("n", [1, 3, 8, 24, 48, 59, 60, 61, 64])
 def test_inner_products_all_pairs(E, n):
+    # n <= 60 takes the packed restrictions and loop (row 60 the end row),
+    # n > 60 the general compiled path
     states = _mixed_states(n, 40 + n)
     pk = [to_packed(s) for s in states]
     A = np.array([pk[a] for a in range(len(pk)) for b in range(len(pk))])
