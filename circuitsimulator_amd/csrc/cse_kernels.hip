@@ -715,7 +715,7 @@ DEV int phiL_unpack(int w) {
     "v_mov_b32 v40, 0xc0000000\n\t" \
     "v_mov_b32 v41, 0x80000000\n\t" \
     "s_mov_b32 s61, m0\n\t" \
-    "s_mov_b32 m0, 0\n\t" \
+    "s_add_u32 m0, %[nb], -1\n\t" \
     "s_nop 0\n" \
     "L_ip%=:\n\t" \
     "v_readlane_b32 s46, %[xtlo], m0\n\t" \
@@ -750,9 +750,8 @@ DEV int phiL_unpack(int w) {
     "s_cselect_b32 s60, s60, 0\n" \
     "L_emit%=:\n\t" \
     "v_writelane_b32 %[parked], s60, m0\n\t" \
-    "s_add_u32 m0, m0, 1\n\t" \
-    "s_cmp_lt_u32 m0, %[nb]\n\t" \
-    "s_cbranch_scc1 L_ip%=\n\t" \
+    "s_sub_u32 m0, m0, 1\n\t" \
+    "s_cbranch_scc0 L_ip%=\n\t" \
     "s_branch L_done%=\n" \
     "L_zero%=:\n\t" \
     "s_mov_b32 s60, 0\n\t" \
