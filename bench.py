@@ -38,7 +38,7 @@ CANONICAL_WORDOPS_PER_IP = 14743
 # --pmc over one dispatch of 2^28 inner products
 # (profiles/k_sweep_norm_estimation_t48_r24_k12_M65536.md):
 VALU_INSTS_PER_IP = 163.8    # SQ_INSTS_VALU / inner products (wave64 instructions)
-SALU_INSTS_PER_IP = 160.1    # SQ_INSTS_SALU / inner products
+SALU_INSTS_PER_IP = 159.1    # SQ_INSTS_SALU / inner products
 VALU_LANE_OPS_PER_IP = VALU_INSTS_PER_IP * 64
 # HBM bytes per inner product of the sweep: (FETCH_SIZE x 2.000 + WRITE_SIZE x
 # 1.000) KiB, factors from a 1 GiB copy kernel, / inner products
