@@ -910,6 +910,30 @@ int cse_random_states(cse_ctx *ctx, int n, uint64_t seed, int64_t first, size_t 
 
 // Sampled-mode instances [0, ninst): one record launch and one sweep over all
 // of them, then each instance's estimator from its own samples.
+// One instance's estimators over count samples: fixed parts of kEstPart
+// samples (so the Kahan mean depends on count alone), spread over nthr host
+// threads.  At 2^16 samples one thread takes about 0.5 ms.
+constexpr int64_t kEstPart = 8192;
+static void split_estimators(int t, int64_t count, const long long *z, const int32_t *se, const double *kz,
+                             int nthr, double *mean, double *mean_kahan) {
+    const int nparts = (int)((count + kEstPart - 1) / kEstPart);
+    if (nparts < 2) {
+        cse_sampled_estimators(t, count, z, se, kz, mean, mean_kahan);
+        return;
+    }
+    std::vector<cse_est_part> parts((size_t)nparts);
+    nthr = std::max(1, std::min(nthr, nparts));
+    auto work = [&](int w) {
+        for (int q = w; q < nparts; q += nthr)
+            cse_est_accumulate(&parts[(size_t)q], t, q * kEstPart, std::min(count, (q + 1) * kEstPart), z, se, kz);
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nthr; w++) pool.emplace_back(work, w);
+    work(0);
+    for (std::thread &th : pool) th.join();
+    cse_est_finish(parts.data(), nparts, t, count, mean, mean_kahan);
+}
+
 static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst, int chunk,
                          cse_sampled_result *res, double *kernel_ms) {
     PlanStore store(ninst);
@@ -952,6 +976,11 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
     const double pms = graphed ? 0.0 : elapsed(ctx->e0, ctx->e1);
     const double kms = graphed ? elapsed(ctx->e0, ctx->e2) : elapsed(ctx->e1, ctx->e2);
     if (kernel_ms) *kernel_ms = kms;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nthr = (ninst >= 8 && pl.nrec >= (1 << 16)) ? (int)std::min(8u, hw ? hw : 1u) : 1;
+    // with one thread per group of instances, each instance's samples take
+    // one thread; otherwise a large instance splits its samples
+    const int sthr = nthr > 1 ? 1 : (int)std::min(8u, hw ? hw : 1u);
     auto convert = [&](int lo, int hi) {
         for (int i = lo; i < hi; i++) {
             cse_sampled_result *r = &res[i];
@@ -962,8 +991,8 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
             r->vanished = 0;
             for (size_t s = 0; s < cnt; s++)
                 if (se[off + s] < 0) r->vanished++;
-            cse_sampled_estimators(in[i].P.nqubits, (int64_t)cnt, z + 4 * off, se + off, kz + 2 * off, &r->mean,
-                                   &r->mean_kahan);
+            split_estimators(in[i].P.nqubits, (int64_t)cnt, z + 4 * off, se + off, kz + 2 * off,
+                             cnt >= 4 * (size_t)kEstPart ? sthr : 1, &r->mean, &r->mean_kahan);
             if (r->zsum) memcpy(r->zsum, z + 4 * off, 32 * cnt);
             if (r->sexp) memcpy(r->sexp, se + off, 4 * cnt);
             if (r->kahan) memcpy(r->kahan, kz + 2 * off, 16 * cnt);
@@ -971,8 +1000,6 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
     };
     // the estimators cost about as much per sample as the copies: split large
     // batches of instances over host threads
-    const unsigned hw = std::thread::hardware_concurrency();
-    const int nthr = (ninst >= 8 && pl.nrec >= (1 << 16)) ? (int)std::min(8u, hw ? hw : 1u) : 1;
     if (nthr > 1) {
         std::vector<std::thread> pool;
         const int per = (ninst + nthr - 1) / nthr;

@@ -188,38 +188,56 @@ static cse_dd dd_split_sum(__int128 hi, unsigned __int128 lo, int e)
     return cse_dd_add(dd_scale(cse_dd_from_i128(top), e + 64), dd_scale(cse_dd_from_i128(low), e));
 }
 
-void cse_sampled_estimators(int t, int64_t count, const long long *z, const int32_t *sexp, const double *kz,
-                            double *mean, double *mean_kahan)
+void cse_est_accumulate(cse_est_part *part, int t, int64_t lo, int64_t hi, const long long *z,
+                        const int32_t *sexp, const double *kz)
 {
     /* exact sums of A and B per projection exponent s (0..64): each
      * __int128 term is split at 2^64, the high parts summed signed and the
      * low parts unsigned (no overflow below 2^63 samples) */
-    __int128 ahi[65] = {0}, bhi[65] = {0};
-    unsigned __int128 alo[65] = {0}, blo[65] = {0};
+    memset(part, 0, sizeof(*part));
+    /* 2^(t - s) for t, s in [0, 64]: a product with a power of two is
+     * exact, so this equals ldexp */
+    double p2[129];
+    for (int e = 0; e <= 128; e++) p2[e] = ldexp(1.0, e - 64);
     double kacc = 0, kcomp = 0;
-    for (int64_t s = 0; s < count; s++) {
+    for (int64_t s = lo; s < hi; s++) {
         const int se = sexp[s];
         if (se < 0 || se > 64) continue;
         const __int128 c0 = z[4 * s], c1 = z[4 * s + 1], c2 = z[4 * s + 2], c3 = z[4 * s + 3];
         const __int128 A = c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3;
         const __int128 B = c0 * c1 - c0 * c3 + c1 * c2 + c2 * c3;
-        ahi[se] += A >> 64;
-        alo[se] += (uint64_t)A;
-        bhi[se] += B >> 64;
-        blo[se] += (uint64_t)B;
+        part->ahi[se] += A >> 64;
+        part->alo[se] += (uint64_t)A;
+        part->bhi[se] += B >> 64;
+        part->blo[se] += (uint64_t)B;
         /* FP32-Kahan path: |S|^2 of the FP32 sums, summed compensated in FP64 */
         double kr = kz[2 * s], ki = kz[2 * s + 1];
-        double v = ldexp(kr * kr + ki * ki, t - se);
+        double v = (kr * kr + ki * ki) * p2[t - se + 64];
         double yk = v - kcomp;
         double tk = kacc + yk;
         kcomp = (tk - kacc) - yk;
         kacc = tk;
     }
+    part->kacc = kacc;
+    part->kcomp = kcomp;
+}
+
+void cse_est_finish(const cse_est_part *parts, int nparts, int t, int64_t count, double *mean,
+                    double *mean_kahan)
+{
     cse_dd X = {0, 0}, Y = {0, 0};
     for (int se = 0; se <= 64; se++) {
+        __int128 ahi = 0, bhi = 0;
+        unsigned __int128 alo = 0, blo = 0;
+        for (int i = 0; i < nparts; i++) {
+            ahi += parts[i].ahi[se];
+            alo += parts[i].alo[se];
+            bhi += parts[i].bhi[se];
+            blo += parts[i].blo[se];
+        }
         const int e = t - se - 64;
-        X = cse_dd_add(X, dd_split_sum(ahi[se], alo[se], e));
-        Y = cse_dd_add(Y, dd_split_sum(bhi[se], blo[se], e));
+        X = cse_dd_add(X, dd_split_sum(ahi, alo, e));
+        Y = cse_dd_add(Y, dd_split_sum(bhi, blo, e));
     }
     cse_dd tot = cse_dd_add(X, cse_dd_mul(Y, SQRT2));
     /* division by count: count is an integer < 2^53, refine once */
@@ -228,7 +246,23 @@ void cse_sampled_estimators(int t, int64_t count, const long long *z, const int3
     cse_dd rem = cse_dd_add(tot, (cse_dd){-prod.hi, -prod.lo});
     q += (rem.hi + rem.lo) / (double)count;
     *mean = q;
+    /* the parts' compensated sums, added compensated in part order */
+    double kacc = parts[0].kacc, kcomp = parts[0].kcomp;
+    for (int i = 1; i < nparts; i++) {
+        double yk = (parts[i].kacc - parts[i].kcomp) - kcomp;
+        double tk = kacc + yk;
+        kcomp = (tk - kacc) - yk;
+        kacc = tk;
+    }
     *mean_kahan = kacc / (double)count;
+}
+
+void cse_sampled_estimators(int t, int64_t count, const long long *z, const int32_t *sexp, const double *kz,
+                            double *mean, double *mean_kahan)
+{
+    cse_est_part part;
+    cse_est_accumulate(&part, t, 0, count, z, sexp, kz);
+    cse_est_finish(&part, 1, t, count, mean, mean_kahan);
 }
 
 /* ------------------------------------------------------------- rank */

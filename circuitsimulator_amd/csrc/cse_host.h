@@ -23,6 +23,21 @@ void cse_dimension_thresholds(int n, uint64_t *thr);
 void cse_sampled_estimators(int t, int64_t count, const long long *z, const int32_t *sexp, const double *kz,
                             double *mean, double *mean_kahan);
 
+/* The same in parts, for splitting the samples over threads: each part
+ * holds the exact sums of A and B per projection exponent over a range of
+ * samples and a compensated sum of the FP32-Kahan estimates.  The exact
+ * sums of the parts add exactly; the compensated sums are added in part
+ * order, so the Kahan mean depends on the split, never on the threads. */
+typedef struct {
+    __int128 ahi[65], bhi[65];
+    unsigned __int128 alo[65], blo[65];
+    double kacc, kcomp;
+} cse_est_part;
+void cse_est_accumulate(cse_est_part *part, int t, int64_t lo, int64_t hi, const long long *z,
+                        const int32_t *sexp, const double *kz);
+void cse_est_finish(const cse_est_part *parts, int nparts, int t, int64_t count, double *mean,
+                    double *mean_kahan);
+
 /* GF(2) rank of k rows of t bits (matrix.c:676-724 semantics) */
 int cse_gf2_rank(const uint64_t *rows, int k);
 

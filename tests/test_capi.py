@@ -137,6 +137,41 @@ def test_sampled_estimator_sum_is_exact():
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libcse.so not built")
+def test_sampled_estimator_parts_add_exactly():
+    """The estimator in parts (the engine splits large sample counts over
+    host threads): the exact mean of any split equals the one-part mean bit
+    for bit, since the parts hold exact integer sums; the Kahan mean agrees
+    to rounding.  The inputs take A and B across 2^64 in both signs."""
+    np = __import__("numpy")
+    lib = ctypes.CDLL(LIB)
+    rng = np.random.default_rng(11)
+    t, count = 48, 20000
+    scale = rng.integers(0, 62, size=(count, 1))
+    z = (rng.integers(-(1 << 62), 1 << 62, size=(count, 4)) >> scale).astype(np.int64).reshape(-1)
+    z[:4] = [(1 << 62) - 1, -(1 << 62), (1 << 62) - 1, (1 << 62) - 1]
+    sexp = rng.integers(-1, 49, size=count).astype(np.int32)
+    kz = rng.standard_normal(2 * count) * 2.0 ** 20
+    P = ctypes.POINTER
+    args = (z.ctypes.data_as(P(ctypes.c_longlong)), sexp.ctypes.data_as(P(ctypes.c_int32)),
+            kz.ctypes.data_as(P(ctypes.c_double)))
+    m1, k1 = ctypes.c_double(), ctypes.c_double()
+    lib.cse_sampled_estimators(ctypes.c_int(t), ctypes.c_int64(count), *args, ctypes.byref(m1), ctypes.byref(k1))
+    part_bytes = 4 * 65 * 16 + 16
+    for cuts in ([0, 8192, 16384, count], [0, 1, 19999, count], [0, 5000, 5000, 12000, count]):
+        nparts = len(cuts) - 1
+        raw = np.zeros(nparts * part_bytes + 64, dtype=np.uint8)
+        base = raw.ctypes.data + (-raw.ctypes.data) % 64
+        for q in range(nparts):
+            lib.cse_est_accumulate(ctypes.c_void_p(base + q * part_bytes), ctypes.c_int(t), ctypes.c_int64(cuts[q]),
+                                   ctypes.c_int64(cuts[q + 1]), *args)
+        m2, k2 = ctypes.c_double(), ctypes.c_double()
+        lib.cse_est_finish(ctypes.c_void_p(base), ctypes.c_int(nparts), ctypes.c_int(t), ctypes.c_int64(count),
+                           ctypes.byref(m2), ctypes.byref(k2))
+        assert m2.value == m1.value
+        assert abs(k2.value - k1.value) <= 1e-13 * abs(k1.value)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libcse.so not built")
 def test_hist_value_is_exact():
     """cse_hist_value: sum over bins (P, M) of count * 2^{P/2} w^M, rounded
     once.  Checked against rational arithmetic (the sqrt2 parts at 60
