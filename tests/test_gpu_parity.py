@@ -158,6 +158,33 @@ def test_sampled_small_all_terms(E, t, r, k, exact):
     assert abs(res.mean_kahan - want) <= 1e-5 * max(1.0, abs(want))
 
 
+def test_sampled_estimators_of_many_samples(E):
+    """40,000 samples: the engine splits the estimators into parts of 8,192
+    samples over host threads.  The int64 mean equals the one-pass host
+    routine on the same per-sample sums bit for bit, and the exact value to
+    one rounding; the Kahan mean agrees to FP32 accumulation."""
+    from circuitsimulator_amd.engine import LIB_PATH, Projector
+    rng = np.random.default_rng(4242)
+    t, r, k, count = 16, 6, 4, 40000
+    ph, xs, zs = random_commuting_projector(rng, t, r)
+    L = [orc.rand_bits(rng, t) for _ in range(k)]
+    res, ex = E.sampled_projector(Projector.make(ph, xs, zs, t), L, k, 0, 99, 0, count)
+    P = ctypes.POINTER
+    zsum = np.ascontiguousarray(ex["zsum"]).reshape(-1)
+    sexp = np.ascontiguousarray(ex["sexp"])
+    kz = np.ascontiguousarray(ex["kahan"]).reshape(-1)
+    m1, k1 = ctypes.c_double(), ctypes.c_double()
+    ctypes.CDLL(LIB_PATH).cse_sampled_estimators(
+        ctypes.c_int(t), ctypes.c_int64(count), zsum.ctypes.data_as(P(ctypes.c_longlong)),
+        sexp.ctypes.data_as(P(ctypes.c_int32)), kz.ctypes.data_as(P(ctypes.c_double)), ctypes.byref(m1),
+        ctypes.byref(k1))
+    assert res.mean == m1.value
+    want = _exact_mean(t, sexp, ex["zsum"])
+    assert abs(res.mean - want) <= 1e-15 * max(1.0, abs(want))
+    assert abs(res.mean_kahan - want) <= 1e-5 * max(1.0, abs(want))
+    assert abs(res.mean_kahan - k1.value) <= 1e-12 * abs(k1.value)
+
+
 @pytest.mark.parametrize("t,r,exact,k", [(1, 1, 1, 0), (3, 2, 1, 0), (6, 3, 1, 0), (5, 2, 0, 3), (9, 4, 0, 4),
                                          (8, 8, 1, 0)])
 def test_exact_all_terms(E, t, r, exact, k):
