@@ -910,6 +910,21 @@ int cse_random_states(cse_ctx *ctx, int n, uint64_t seed, int64_t first, size_t 
 
 // Sampled-mode instances [0, ninst): one record launch and one sweep over all
 // of them, then each instance's estimator from its own samples.
+// Runs job(w) for w in [0, nw): w >= 1 on threads of their own, w = 0 on the
+// caller's.  A thread that cannot be started leaves its job to the caller.
+extern "C++" template <class F>
+static void run_jobs(int nw, F &&job) {
+    std::vector<std::thread> pool;
+    int w = 1;
+    try {
+        for (; w < nw; w++) pool.emplace_back(job, w);
+    } catch (...) {
+    }
+    for (int v = w; v < nw; v++) job(v);
+    job(0);
+    for (std::thread &th : pool) th.join();
+}
+
 // One instance's estimators over count samples: fixed parts of kEstPart
 // samples (so the Kahan mean depends on count alone), spread over nthr host
 // threads.  At 2^16 samples one thread takes about 0.5 ms.
@@ -927,10 +942,7 @@ static void split_estimators(int t, int64_t count, const long long *z, const int
         for (int q = w; q < nparts; q += nthr)
             cse_est_accumulate(&parts[(size_t)q], t, q * kEstPart, std::min(count, (q + 1) * kEstPart), z, se, kz);
     };
-    std::vector<std::thread> pool;
-    for (int w = 1; w < nthr; w++) pool.emplace_back(work, w);
-    work(0);
-    for (std::thread &th : pool) th.join();
+    run_jobs(nthr, work);
     cse_est_finish(parts.data(), nparts, t, count, mean, mean_kahan);
 }
 
@@ -1001,13 +1013,8 @@ static int sampled_batch(cse_ctx *ctx, const cse_sampled_instance *in, int ninst
     // the estimators cost about as much per sample as the copies: split large
     // batches of instances over host threads
     if (nthr > 1) {
-        std::vector<std::thread> pool;
         const int per = (ninst + nthr - 1) / nthr;
-        for (int w = 0; w < nthr; w++) {
-            const int lo = w * per, hi = std::min(ninst, lo + per);
-            if (lo < hi) pool.emplace_back(convert, lo, hi);
-        }
-        for (std::thread &th : pool) th.join();
+        run_jobs(nthr, [&](int w) { convert(std::min(ninst, w * per), std::min(ninst, (w + 1) * per)); });
     } else {
         convert(0, ninst);
     }
@@ -1127,13 +1134,8 @@ static int exact_batch(cse_ctx *ctx, const cse_instance *in, int ninst, int chun
     const unsigned hw = std::thread::hardware_concurrency();
     const int nthr = ninst >= 512 ? (int)std::min(8u, hw ? hw : 1u) : 1;
     if (nthr > 1) {
-        std::vector<std::thread> pool;
         const int per = (ninst + nthr - 1) / nthr;
-        for (int w = 0; w < nthr; w++) {
-            const int lo = w * per, hi = std::min(ninst, lo + per);
-            if (lo < hi) pool.emplace_back(convert, lo, hi);
-        }
-        for (std::thread &th : pool) th.join();
+        run_jobs(nthr, [&](int w) { convert(std::min(ninst, w * per), std::min(ninst, (w + 1) * per)); });
     } else {
         convert(0, ninst);
     }
