@@ -577,8 +577,8 @@ def main():
         args.warmup = 20 if args.config == "c1" else 1
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this engine is measured on one GPU; multi-GPU runs are not part of this build "
-              "(the per-GPU sharding design is described in the design notes).", file=sys.stderr)
+        print("bench.py: the engine runs on one GPU; --gpus above 1 is not supported (a call's work is not "
+              "split across GPUs).", file=sys.stderr)
         sys.exit(2)
 
     if args.config != "c4":

@@ -118,8 +118,11 @@ void orc_zomega_add(int64_t c[4], int E, int eps, int p, int m);
 void orc_zomega_norm2(const int64_t c[4], __int128 *A, __int128 *B);
 
 /* ---- op-count model of the canonical dense-row algorithm ---- */
-/* Counts 64-bit word operations of one inner product of (s1, s2) under the
- * dense-row cost model documented in the engine's design notes. */
+/* Counts 64-bit word operations of one inner product of (s1, s2) under a
+ * dense-row cost model of innerProductExact (stabilizer.c:589-659): 9 k1 per
+ * shrink by a row of s2's dual basis (until one empties the state), 6 k1 k2
+ * for the basis alignment and 8 k1^2 for the exponential sum, with k1 the
+ * dimension after the shrinks. */
 long orc_ip_wordops(const orc_state *s1, const orc_state *s2);
 
 /* ---- sampled-mode sweep over given projected thetas (innerprod.c:119-139) ----
