@@ -550,7 +550,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 3; c1: 1,000 probability evaluations of about 0.13 ms each; c4: 20; "
+                    help="timed steps (default 3; c1: 1,000 probability evaluations of about 0.13 ms each; c4: 60; "
                          "c5: runs of --shots shots)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1; c1: 20)")
     ap.add_argument("--samples", type=int, default=65536, help="random thetas per step (M)")
@@ -576,7 +576,7 @@ def main():
                     help="sweep kernel of the configuration's mode (cse_set_sweep_kernel; 0: the engine's choice)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"c1": 1000, "c4": 20}.get(args.config, 3)
+        args.steps = {"c1": 1000, "c4": 60}.get(args.config, 3)
     if args.warmup is None:
         args.warmup = 20 if args.config == "c1" else 1
     world = int(os.environ.get("WORLD_SIZE", "1"))
